@@ -1,0 +1,423 @@
+// Host driver and C ABI of libgmms.so (declarations: include/gpu_mms/gmms.h).
+//
+// The round schedule follows the reference driver src/multimergesort.hxx:
+// 56-141: base case over 1024-element tiles, then
+// `for (L = 1024; L < n; L *= K)` rounds, tasks = ceil(n / (K*L)) (:89),
+// buffers ping-pong (:128) and the caller learns which one holds the result
+// (:140). Differences by design: the workspace is caller-owned (no
+// cudaMalloc in the timed path), sizes are 64-bit, all launches go to the
+// caller's stream, and errors come back as status codes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/gpu_mms/gmms.h"
+#include "gmms_basecase.hxx"
+#include "gmms_merge.hxx"
+
+using namespace gmms;
+
+namespace {
+
+struct Resolved {
+  int K, PL, D, mergers;
+  int64_t slack;
+};
+
+int ilog2(int x) {
+  int r = 0;
+  while ((1 << r) < x) ++r;
+  return r;
+}
+
+int cu_count() {
+  static int cached = 0;
+  static std::mutex mu;
+  std::lock_guard<std::mutex> g(mu);
+  if (cached) return cached;
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
+  cached = cus;
+  return cached;
+}
+
+int elem_words(int dtype) {
+  switch (dtype) {
+    case GMMS_INT32: case GMMS_UINT32: case GMMS_FLOAT32: return 1;
+    case GMMS_INT64: case GMMS_UINT64: case GMMS_FLOAT64: return 2;
+    case GMMS_PAIR_U64_U32: return 3;
+    default: return 0;
+  }
+}
+bool valid_dtype(int dtype) { return elem_words(dtype) > 0; }
+bool has_vals(int dtype) { return dtype == GMMS_PAIR_U64_U32; }
+
+constexpr size_t LDS_LIMIT = 160 * 1024;  // gfx950 LDS per workgroup
+
+// K = 0 means: the largest power of two <= 64 whose heap + rings fit in LDS
+int resolve(int dtype, const gmms_params* p, Resolved* r, bool need_device) {
+  gmms_params d;
+  gmms_get_default_params(&d);
+  const gmms_params& q = p ? *p : d;
+  int D = q.prefetch ? q.prefetch : d.prefetch;
+  if (D < 1 || D > 8) return GMMS_ERROR_INVALID_VALUE;
+  const int W = elem_words(dtype);
+  int K = q.K;
+  if (K == 0) {
+    K = MAX_K;
+    while (K > 2 && merge_lds_words(K, W, D) * sizeof(uint32_t) > LDS_LIMIT) K >>= 1;
+  }
+  if (K < 2 || K > MAX_K || (K & (K - 1))) return GMMS_ERROR_INVALID_VALUE;
+  if (merge_lds_words(K, W, D) * sizeof(uint32_t) > LDS_LIMIT) return GMMS_ERROR_UNSUPPORTED;
+  if (q.mergers < 0 || q.slack < 0) return GMMS_ERROR_INVALID_VALUE;
+  r->K = K;
+  r->PL = ilog2(K);
+  r->D = D;
+  r->mergers = q.mergers ? q.mergers : (need_device ? 2 * cu_count() : 512);
+  r->slack = q.slack;
+  return GMMS_SUCCESS;
+}
+
+// mergers per task for a round
+int mergers_per_task(int64_t n, int64_t L, int K, int target) {
+  const int64_t TK = (int64_t)K * L;
+  const int64_t tasks = (n + TK - 1) / TK;
+  if (tasks >= target) return 1;
+  int64_t m = (target + tasks - 1) / tasks;
+  // keep every merger at least a few heap-builds worth of output
+  const int64_t minsz = (int64_t)WAVE * K * 4;
+  const int64_t tsz = std::min<int64_t>(n, TK);
+  m = std::min<int64_t>(m, std::max<int64_t>(1, tsz / minsz));
+  return (int)std::max<int64_t>(1, m);
+}
+
+int64_t slack_for(const Resolved& r, int64_t n, int64_t L, int m) {
+  if (r.slack) return r.slack;
+  const int64_t TK = (int64_t)r.K * L;
+  const int64_t tsz = std::min<int64_t>(n, TK);
+  return std::max<int64_t>(WAVE, tsz / m / 32);
+}
+
+size_t ws_bytes_for(int64_t n, const Resolved& r) {
+  size_t need = 0;
+  for (int64_t L = TILE; L < n; L *= r.K) {
+    const int64_t TK = (int64_t)r.K * L;
+    const int64_t tasks = (n + TK - 1) / TK;
+    const int m = mergers_per_task(n, L, r.K, r.mergers);
+    if (m > 1) {
+      const size_t b = (size_t)(tasks * m);
+      need = std::max(need, b * (size_t)(r.K + 1) * sizeof(int64_t));
+    }
+  }
+  return need;
+}
+
+#define HIP_OK(x)                                  \
+  do {                                             \
+    if ((x) != hipSuccess) return GMMS_ERROR_HIP;  \
+  } while (0)
+
+template <int DT>
+int launch_basecase(void* keys, uint32_t* vals, int64_t n, bool cin, bool cout, hipStream_t s) {
+  if (n <= 0) return GMMS_SUCCESS;
+  const int64_t tiles = (n + TILE - 1) / TILE;
+  const dim3 grid((unsigned)((tiles + BASE_WAVES - 1) / BASE_WAVES));
+  const dim3 block(BASE_WAVES * WAVE);
+  if (cin && cout) hipLaunchKernelGGL((k_basecase<DT, true, true>), grid, block, 0, s, keys, vals, n);
+  else if (cin) hipLaunchKernelGGL((k_basecase<DT, true, false>), grid, block, 0, s, keys, vals, n);
+  else if (cout) hipLaunchKernelGGL((k_basecase<DT, false, true>), grid, block, 0, s, keys, vals, n);
+  else hipLaunchKernelGGL((k_basecase<DT, false, false>), grid, block, 0, s, keys, vals, n);
+  HIP_OK(hipGetLastError());
+  return GMMS_SUCCESS;
+}
+
+template <int DT>
+int launch_partition(RoundArgs& a, hipStream_t s) {
+  const int64_t waves = a.tasks * (a.m - 1);
+  if (waves <= 0) return GMMS_SUCCESS;
+  const dim3 grid((unsigned)((waves + PART_WAVES - 1) / PART_WAVES));
+  hipLaunchKernelGGL((k_partition<DT>), grid, dim3(PART_WAVES * WAVE), 0, s, a);
+  HIP_OK(hipGetLastError());
+  return GMMS_SUCCESS;
+}
+
+template <int DT>
+int launch_merge(RoundArgs& a, bool cout, hipStream_t s) {
+  using E = typename DType<DT>::E;
+  const size_t lds = merge_lds_words(a.K, E::W, a.D) * sizeof(uint32_t);
+  if (lds > LDS_LIMIT) return GMMS_ERROR_UNSUPPORTED;
+  const dim3 grid((unsigned)(a.tasks * a.m));
+  const dim3 block(a.PL * WAVE);
+  if (cout) {
+    static bool attr = false;
+    if (!attr) {
+      (void)hipFuncSetAttribute((const void*)k_merge<DT, true>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      attr = true;
+    }
+    hipLaunchKernelGGL((k_merge<DT, true>), grid, block, lds, s, a);
+  } else {
+    static bool attr = false;
+    if (!attr) {
+      (void)hipFuncSetAttribute((const void*)k_merge<DT, false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      attr = true;
+    }
+    hipLaunchKernelGGL((k_merge<DT, false>), grid, block, lds, s, a);
+  }
+  HIP_OK(hipGetLastError());
+  return GMMS_SUCCESS;
+}
+
+RoundArgs round_args(const void* ik, const void* iv, void* ok, void* ov, int64_t n, int64_t L,
+                     const Resolved& r, int m, void* ws) {
+  RoundArgs a{};
+  a.in_keys = ik;
+  a.in_vals = (const uint32_t*)iv;
+  a.out_keys = ok;
+  a.out_vals = (uint32_t*)ov;
+  a.n = n;
+  a.L = L;
+  a.K = r.K;
+  a.PL = r.PL;
+  a.D = r.D;
+  a.m = m;
+  const int64_t TK = (int64_t)r.K * L;
+  a.tasks = (n + TK - 1) / TK;
+  a.slack = slack_for(r, n, L, m);
+  const int64_t b = a.tasks * m;
+  a.cuts = (int64_t*)ws;
+  a.skips = ws ? (int64_t*)ws + b * r.K : nullptr;
+  return a;
+}
+
+template <int DT>
+int run_round(const void* ik, const void* iv, void* ok, void* ov, int64_t n, int64_t L, const Resolved& r,
+              bool cout, void* ws, size_t wsb, hipStream_t s) {
+  const int m = mergers_per_task(n, L, r.K, r.mergers);
+  RoundArgs a = round_args(ik, iv, ok, ov, n, L, r, m, ws);
+  if (m > 1) {
+    const size_t need = (size_t)(a.tasks * m) * (size_t)(r.K + 1) * sizeof(int64_t);
+    if (!ws || wsb < need) return GMMS_ERROR_WORKSPACE;
+    int st = launch_partition<DT>(a, s);
+    if (st) return st;
+  }
+  return launch_merge<DT>(a, cout, s);
+}
+
+template <int DT>
+int run_sort(void* k0, void* k1, void* v0, void* v1, int64_t n, const Resolved& r, void* ws, size_t wsb,
+             hipStream_t s, int* alt) {
+  int rounds = 0;
+  for (int64_t L = TILE; L < n; L *= r.K) ++rounds;
+  int st = launch_basecase<DT>(k0, (uint32_t*)v0, n, true, rounds == 0, s);
+  if (st) return st;
+  void* kb[2] = {k0, k1};
+  void* vb[2] = {v0, v1};
+  int cur = 0, done = 0;
+  for (int64_t L = TILE; L < n; L *= r.K, ++done) {
+    st = run_round<DT>(kb[cur], vb[cur], kb[cur ^ 1], vb[cur ^ 1], n, L, r, done == rounds - 1, ws, wsb, s);
+    if (st) return st;
+    cur ^= 1;
+  }
+  if (alt) *alt = cur;
+  return GMMS_SUCCESS;
+}
+
+template <class F>
+int dispatch(int dtype, F&& f) {
+  switch (dtype) {
+    case GMMS_INT32: return f(std::integral_constant<int, DT_I32>{});
+    case GMMS_UINT32: return f(std::integral_constant<int, DT_U32>{});
+    case GMMS_FLOAT32: return f(std::integral_constant<int, DT_F32>{});
+    case GMMS_INT64: return f(std::integral_constant<int, DT_I64>{});
+    case GMMS_UINT64: return f(std::integral_constant<int, DT_U64>{});
+    case GMMS_FLOAT64: return f(std::integral_constant<int, DT_F64>{});
+    case GMMS_PAIR_U64_U32: return f(std::integral_constant<int, DT_PAIR>{});
+    default: return GMMS_ERROR_INVALID_VALUE;
+  }
+}
+
+template <int DT>
+__global__ void k_pad(void* keys, uint32_t* vals, int64_t count) {
+  using E = typename DType<DT>::E;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) gstore<DT, true>(keys, vals, i, Limits<E>::hi());
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gmms_version(void) { return "gpu-mms-amd 0.1.0 gfx950"; }
+
+const char* gmms_status_string(int s) {
+  switch (s) {
+    case GMMS_SUCCESS: return "success";
+    case GMMS_ERROR_INVALID_VALUE: return "invalid value";
+    case GMMS_ERROR_UNSUPPORTED: return "unsupported";
+    case GMMS_ERROR_WORKSPACE: return "workspace missing or too small";
+    case GMMS_ERROR_HIP: return "HIP runtime error";
+    case GMMS_ERROR_OUT_OF_MEMORY: return "out of device memory";
+    case GMMS_ERROR_BUDGET: return "request exceeds the HBM budget";
+    default: return "unknown status";
+  }
+}
+
+int gmms_get_default_params(gmms_params* p) {
+  if (!p) return GMMS_ERROR_INVALID_VALUE;
+  std::memset(p, 0, sizeof(*p));
+  p->K = 0;  // auto: 64 for 4- and 8-byte keys, 32 for pairs
+  p->pipeline = 1;
+  p->mergers = 0;
+  p->prefetch = 2;
+  p->slack = 0;
+  return GMMS_SUCCESS;
+}
+
+int gmms_element_size(int dtype, size_t* kb, size_t* vb) {
+  if (!valid_dtype(dtype) || !kb || !vb) return GMMS_ERROR_INVALID_VALUE;
+  const int w = elem_words(dtype);
+  *kb = w == 1 ? 4 : 8;
+  *vb = has_vals(dtype) ? 4 : 0;
+  return GMMS_SUCCESS;
+}
+
+int gmms_num_rounds(int64_t n, int K, int* rounds) {
+  if (!rounds || n < 0 || K < 2) return GMMS_ERROR_INVALID_VALUE;
+  int r = 0;
+  for (int64_t L = TILE; L < n; L *= K) ++r;
+  *rounds = r;
+  return GMMS_SUCCESS;
+}
+
+int gmms_workspace_size(int dtype, int64_t n, const gmms_params* p, size_t* bytes) {
+  if (!valid_dtype(dtype) || !bytes || n < 0) return GMMS_ERROR_INVALID_VALUE;
+  Resolved r;
+  int st = resolve(dtype, p, &r, true);
+  if (st) return st;
+  *bytes = ws_bytes_for(n, r);
+  return GMMS_SUCCESS;
+}
+
+int gmms_round_workspace_size(int dtype, int64_t n, int64_t L, const gmms_params* p, size_t* bytes) {
+  if (!valid_dtype(dtype) || !bytes || n < 0 || L < 1) return GMMS_ERROR_INVALID_VALUE;
+  Resolved r;
+  int st = resolve(dtype, p, &r, true);
+  if (st) return st;
+  *bytes = 0;
+  if (n == 0) return GMMS_SUCCESS;
+  const int m = mergers_per_task(n, L, r.K, r.mergers);
+  const int64_t TK = (int64_t)r.K * L;
+  const int64_t tasks = (n + TK - 1) / TK;
+  if (m > 1) *bytes = (size_t)(tasks * m) * (size_t)(r.K + 1) * sizeof(int64_t);
+  return GMMS_SUCCESS;
+}
+
+int gmms_sort(int dtype, void* keys, void* keys_alt, void* vals, void* vals_alt, int64_t n,
+              const gmms_params* p, void* ws, size_t wsb, void* stream, int* alt) {
+  if (!valid_dtype(dtype) || n < 0) return GMMS_ERROR_INVALID_VALUE;
+  if (alt) *alt = 0;
+  if (n == 0) return GMMS_SUCCESS;
+  if (!keys || !keys_alt) return GMMS_ERROR_INVALID_VALUE;
+  if (has_vals(dtype) != (vals != nullptr) || has_vals(dtype) != (vals_alt != nullptr))
+    return GMMS_ERROR_INVALID_VALUE;
+  Resolved r;
+  int st = resolve(dtype, p, &r, true);
+  if (st) return st;
+  if (wsb < ws_bytes_for(n, r) || (ws_bytes_for(n, r) > 0 && !ws)) return GMMS_ERROR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  return dispatch(dtype, [&](auto dt) {
+    return run_sort<decltype(dt)::value>(keys, keys_alt, vals, vals_alt, n, r, ws, wsb, s, alt);
+  });
+}
+
+int gmms_multimergesort(int dtype, void* input, void* output, void* host_array, int thread_blocks, int64_t n,
+                        void** result) {
+  (void)host_array;
+  if (!result) return GMMS_ERROR_INVALID_VALUE;
+  *result = input;
+  if (!valid_dtype(dtype) || n < 0) return GMMS_ERROR_INVALID_VALUE;
+  if (n == 0) return GMMS_SUCCESS;
+  if (has_vals(dtype)) return GMMS_ERROR_UNSUPPORTED;  // pairs need the value buffers: use gmms_sort
+  gmms_params p;
+  gmms_get_default_params(&p);
+  if (thread_blocks > 0) p.mergers = thread_blocks;
+  size_t wsb = 0;
+  int st = gmms_workspace_size(dtype, n, &p, &wsb);
+  if (st) return st;
+  void* ws = nullptr;
+  if (wsb && hipMalloc(&ws, wsb) != hipSuccess) return GMMS_ERROR_OUT_OF_MEMORY;
+  int alt = 0;
+  st = gmms_sort(dtype, input, output, nullptr, nullptr, n, &p, ws, wsb, nullptr, &alt);
+  if (hipDeviceSynchronize() != hipSuccess && !st) st = GMMS_ERROR_HIP;
+  if (ws) (void)hipFree(ws);
+  if (!st) *result = alt ? output : input;
+  return st;
+}
+
+int gmms_squaresort(int dtype, void* keys, void* vals, int64_t n, int to_internal, int to_external, void* stream) {
+  if (!valid_dtype(dtype) || n < 0) return GMMS_ERROR_INVALID_VALUE;
+  if (n == 0) return GMMS_SUCCESS;
+  if (!keys || has_vals(dtype) != (vals != nullptr)) return GMMS_ERROR_INVALID_VALUE;
+  hipStream_t s = (hipStream_t)stream;
+  return dispatch(dtype, [&](auto dt) {
+    return launch_basecase<decltype(dt)::value>(keys, (uint32_t*)vals, n, to_internal != 0, to_external != 0, s);
+  });
+}
+
+int gmms_merge_round(int dtype, const void* ik, const void* iv, void* ok, void* ov, int64_t n, int64_t L,
+                     const gmms_params* p, int to_external, void* ws, size_t wsb, void* stream) {
+  if (!valid_dtype(dtype) || n < 0 || L < 1) return GMMS_ERROR_INVALID_VALUE;
+  if (n == 0) return GMMS_SUCCESS;
+  if (!ik || !ok || has_vals(dtype) != (iv != nullptr) || has_vals(dtype) != (ov != nullptr))
+    return GMMS_ERROR_INVALID_VALUE;
+  Resolved r;
+  int st = resolve(dtype, p, &r, true);
+  if (st) return st;
+  hipStream_t s = (hipStream_t)stream;
+  return dispatch(dtype, [&](auto dt) {
+    return run_round<decltype(dt)::value>(ik, iv, ok, ov, n, L, r, to_external != 0, ws, wsb, s);
+  });
+}
+
+int gmms_find_partitions(int dtype, const void* keys, const void* vals, int64_t n, int64_t L, const gmms_params* p,
+                         int64_t* cuts, int64_t* skips, int64_t cap, int* mpt, void* stream) {
+  if (!valid_dtype(dtype) || n < 0 || L < 1 || !mpt) return GMMS_ERROR_INVALID_VALUE;
+  if (!keys || !cuts || !skips || has_vals(dtype) != (vals != nullptr)) return GMMS_ERROR_INVALID_VALUE;
+  Resolved r;
+  int st = resolve(dtype, p, &r, true);
+  if (st) return st;
+  const int m = mergers_per_task(n, L, r.K, r.mergers);
+  *mpt = m;
+  const int64_t TK = (int64_t)r.K * L;
+  const int64_t tasks = (n + TK - 1) / TK;
+  if (tasks * m > cap) return GMMS_ERROR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  HIP_OK(hipMemsetAsync(cuts, 0, (size_t)(tasks * m * r.K) * sizeof(int64_t), s));
+  HIP_OK(hipMemsetAsync(skips, 0, (size_t)(tasks * m) * sizeof(int64_t), s));
+  if (m <= 1) return GMMS_SUCCESS;
+  RoundArgs a = round_args(keys, vals, nullptr, nullptr, n, L, r, m, nullptr);
+  a.cuts = cuts;
+  a.skips = skips;
+  return dispatch(dtype, [&](auto dt) { return launch_partition<decltype(dt)::value>(a, s); });
+}
+
+int gmms_pad(int dtype, void* keys, void* vals, int64_t count, void* stream) {
+  if (!valid_dtype(dtype) || count < 0) return GMMS_ERROR_INVALID_VALUE;
+  if (count == 0) return GMMS_SUCCESS;
+  if (!keys || has_vals(dtype) != (vals != nullptr)) return GMMS_ERROR_INVALID_VALUE;
+  hipStream_t s = (hipStream_t)stream;
+  return dispatch(dtype, [&](auto dt) {
+    constexpr int DT = decltype(dt)::value;
+    hipLaunchKernelGGL((k_pad<DT>), dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, keys, (uint32_t*)vals,
+                       count);
+    return hipGetLastError() == hipSuccess ? (int)GMMS_SUCCESS : (int)GMMS_ERROR_HIP;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,158 @@
+// Base case: every 1024-element tile sorted by one 64-lane wavefront, in
+// registers. Replaces the reference's squareSort (src/basecase/
+// squareSort.hxx:62-131), which sorts a 32x32 tile with one 32-lane warp via
+// bitonicSort32 + merge32..merge512 (src/basecase/sortRowMajor.hxx:72-220)
+// and selection-sorts the N%1024 tail on one thread (:69-88).
+//
+// Layout: element e of the tile lives in lane e/16, register e%16 (16 keys per
+// lane). The network is bitonic sort in "flip" form (every compare-exchange
+// puts the minimum at the lower index):
+//   * 16 registers per lane: Batcher odd-even merge sort (63 exchanges),
+//   * for block sizes 32..1024: one mirror stage across lanes (lane ^ (2^b-1),
+//     register r <-> 15-r), half-cleaners across lanes at lane distance
+//     2^(b-1)..1 (DPP / ds_swizzle / ds_bpermute), then in-register
+//     half-cleaners at distances 8,4,2,1.
+// The ragged last tile is padded with the maximum element in registers, not
+// in memory (the reference's pad.hxx writes MAXVAL-1 into the array and its
+// sentinel collides with real keys; here a padded slot is simply never stored).
+// Loads are fully coalesced (lane + 64*r); the sorted tile goes back to
+// memory through a bank-padded LDS transpose so the stores are coalesced too.
+#pragma once
+#include "gmms_elem.hxx"
+
+namespace gmms {
+
+constexpr int TILE = 1024;
+constexpr int TPL = TILE / WAVE;  // keys per lane
+
+// Batcher odd-even merge sort on N registers (iterative form, Knuth 5.3.4).
+template <class E, int N>
+__device__ __forceinline__ void oddeven_sort(E (&v)[N]) {
+#pragma unroll
+  for (int p = 1; p < N; p <<= 1) {
+#pragma unroll
+    for (int k = p; k >= 1; k >>= 1) {
+#pragma unroll
+      for (int j = k % p; j + k < N; j += 2 * k) {
+#pragma unroll
+        for (int i = 0; i < k; ++i) {
+          if (i + j + k < N && (i + j) / (2 * p) == (i + j + k) / (2 * p)) cswap(v[i + j], v[i + j + k]);
+        }
+      }
+    }
+  }
+}
+
+// in-register half-cleaners, distances N/2 .. 1
+template <class E, int N>
+__device__ __forceinline__ void clean_regs(E (&v)[N]) {
+#pragma unroll
+  for (int d = N / 2; d >= 1; d >>= 1) {
+#pragma unroll
+    for (int r = 0; r < N; ++r) {
+      if ((r & d) == 0) cswap(v[r], v[r + d]);
+    }
+  }
+}
+
+// mirror stage of block size 16 * 2^(B+1): partner lane ^ (2^(B+1)-1), reg N-1-r
+template <int B, class E, int N>
+__device__ __forceinline__ void flip_lanes(E (&v)[N], int lane) {
+  constexpr int M = (2 << B) - 1;
+  const bool low = ((lane >> B) & 1) == 0;
+  E p[N];
+#pragma unroll
+  for (int r = 0; r < N; ++r) p[r] = xlane<M>(v[N - 1 - r]);
+#pragma unroll
+  for (int r = 0; r < N; ++r) v[r] = low ? emin(v[r], p[r]) : emax(v[r], p[r]);
+}
+
+// half-cleaner across lanes at lane distance D
+template <int D, class E, int N>
+__device__ __forceinline__ void clean_lanes(E (&v)[N], int lane) {
+  const bool low = (lane & D) == 0;
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    E p = xlane<D>(v[r]);
+    v[r] = low ? emin(v[r], p) : emax(v[r], p);
+  }
+}
+
+template <int B, class E, int N>
+__device__ __forceinline__ void lane_cleaners(E (&v)[N], int lane) {
+  if constexpr (B >= 6) clean_lanes<32>(v, lane);
+  if constexpr (B >= 5) clean_lanes<16>(v, lane);
+  if constexpr (B >= 4) clean_lanes<8>(v, lane);
+  if constexpr (B >= 3) clean_lanes<4>(v, lane);
+  if constexpr (B >= 2) clean_lanes<2>(v, lane);
+  if constexpr (B >= 1) clean_lanes<1>(v, lane);
+}
+
+template <int B, class E, int N>
+__device__ __forceinline__ void merge_level(E (&v)[N], int lane) {
+  flip_lanes<B>(v, lane);
+  lane_cleaners<B>(v, lane);
+  clean_regs(v);
+}
+
+// Sort 1024 elements held as v[16] per lane; result: lane l holds ranks
+// 16l .. 16l+15 in ascending register order.
+template <class E>
+__device__ __forceinline__ void sort_tile(E (&v)[TPL], int lane) {
+  oddeven_sort(v);
+  merge_level<0>(v, lane);
+  merge_level<1>(v, lane);
+  merge_level<2>(v, lane);
+  merge_level<3>(v, lane);
+  merge_level<4>(v, lane);
+  merge_level<5>(v, lane);
+}
+
+constexpr int BASE_WAVES = 4;                   // waves (tiles) per workgroup
+constexpr int XPOSE_STRIDE = TPL + 1;            // bank padding
+constexpr int XPOSE_WORDS = WAVE * XPOSE_STRIDE; // per wave per word plane
+
+// In-place base case over keys[0..n): CONV_IN maps raw -> internal on load,
+// CONV_OUT maps internal -> raw on store.
+template <int DT, bool CONV_IN, bool CONV_OUT>
+__global__ __launch_bounds__(BASE_WAVES * WAVE) void k_basecase(void* keys, uint32_t* vals, int64_t n) {
+  using E = typename DType<DT>::E;
+  __shared__ uint32_t xp[BASE_WAVES * E::W * XPOSE_WORDS];
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wave = threadIdx.x / WAVE;
+  const int64_t tile = (int64_t)blockIdx.x * BASE_WAVES + wave;
+  const int64_t base = tile * TILE;
+  if (base >= n) return;  // per-wave work only: no workgroup barrier below
+  const int cnt = (int)((n - base) < TILE ? (n - base) : TILE);
+
+  E v[TPL];
+#pragma unroll
+  for (int r = 0; r < TPL; ++r) {
+    const int idx = r * WAVE + lane;
+    v[r] = idx < cnt ? gload<DT, CONV_IN>(keys, vals, base + idx) : Limits<E>::hi();
+  }
+
+  sort_tile(v, lane);
+
+  // transpose through LDS: lane l's registers hold ranks 16l..16l+15
+  uint32_t* my = xp + wave * E::W * XPOSE_WORDS;
+#pragma unroll
+  for (int w = 0; w < E::W; ++w) {
+#pragma unroll
+    for (int r = 0; r < TPL; ++r) my[w * XPOSE_WORDS + lane * XPOSE_STRIDE + r] = word(v[r], w);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int r = 0; r < TPL; ++r) {
+    const int e = r * WAVE + lane;  // rank
+    const int slot = (e / TPL) * XPOSE_STRIDE + (e % TPL);
+    uint32_t w[E::W];
+#pragma unroll
+    for (int q = 0; q < E::W; ++q) w[q] = my[q * XPOSE_WORDS + slot];
+    if (e < cnt) gstore<DT, CONV_OUT>(keys, vals, base + e, from_words<E>(w));
+  }
+}
+
+}  // namespace gmms

@@ -1,0 +1,194 @@
+// Element types, key-order maps and 64-lane cross-lane exchanges for gfx950.
+//
+// Every kernel works on an "internal" element whose order is plain unsigned
+// order: the comparator contract of the reference (src/cmp.hxx:37-54,
+// cmp(a,b) == "a strictly precedes b", MAXVAL larger than every key) is met
+// by mapping each user dtype to unsigned bits once, on the base-case load, and
+// back once, on the last round's store:
+//   int32/int64  : flip the sign bit
+//   float32/64   : IEEE total order (negatives reversed, -0 before +0)
+//   (u64 key, u32 value) pairs : lexicographic (key, value), no map
+// so ties are broken by the full element bits and the sorted output is unique.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gmms {
+
+constexpr int WAVE = 64;
+
+// dtype codes; must match include/gpu_mms/gmms.h
+enum : int {
+  DT_I32 = 0, DT_U32 = 1, DT_F32 = 2, DT_I64 = 3, DT_U64 = 4, DT_F64 = 5, DT_PAIR = 6
+};
+
+// ------------------------------------------------------------------ words
+// Cross-lane value of lane (lane ^ M) for one 32-bit word. Masks below 64
+// use DPP (quad_perm, row mirrors, row_ror:8) or ds_swizzle inside 32 lanes;
+// 32 and 63 cross the half-wave and go through ds_bpermute.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_mov(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, CTRL, 0xF, 0xF, false);
+}
+template <int PAT>
+__device__ __forceinline__ uint32_t swizzle(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, PAT);
+}
+template <int M>
+__device__ __forceinline__ uint32_t xor_lane(uint32_t x) {
+  static_assert(M > 0 && M < 64, "lane mask");
+  if constexpr (M == 1) return dpp_mov<0xB1>(x);        // quad_perm [1,0,3,2]
+  else if constexpr (M == 2) return dpp_mov<0x4E>(x);   // quad_perm [2,3,0,1]
+  else if constexpr (M == 3) return dpp_mov<0x1B>(x);   // quad_perm [3,2,1,0]
+  else if constexpr (M == 7) return dpp_mov<0x141>(x);  // row_half_mirror
+  else if constexpr (M == 8) return dpp_mov<0x128>(x);  // row_ror:8
+  else if constexpr (M == 15) return dpp_mov<0x140>(x); // row_mirror
+  else if constexpr (M < 32) return swizzle<0x1F | (M << 10)>(x);  // bit-mode xor
+  else return (uint32_t)__shfl_xor((int)x, M, 64);
+}
+
+// ---------------------------------------------------------------- elements
+struct E32 {
+  static constexpr int W = 1;
+  static constexpr bool HAS_VAL = false;
+  uint32_t k;
+};
+struct E64 {
+  static constexpr int W = 2;
+  static constexpr bool HAS_VAL = false;
+  uint64_t k;
+};
+struct P96 {
+  static constexpr int W = 3;
+  static constexpr bool HAS_VAL = true;
+  uint64_t k;
+  uint32_t v;
+};
+
+__device__ __forceinline__ bool lt(const E32& a, const E32& b) { return a.k < b.k; }
+__device__ __forceinline__ bool lt(const E64& a, const E64& b) { return a.k < b.k; }
+__device__ __forceinline__ bool lt(const P96& a, const P96& b) {
+  return a.k < b.k || (a.k == b.k && a.v < b.v);
+}
+__device__ __forceinline__ bool le(const E32& a, const E32& b) { return a.k <= b.k; }
+__device__ __forceinline__ bool le(const E64& a, const E64& b) { return a.k <= b.k; }
+__device__ __forceinline__ bool le(const P96& a, const P96& b) { return !lt(b, a); }
+
+template <class E> __device__ __forceinline__ E emin(const E& a, const E& b) { return lt(b, a) ? b : a; }
+template <class E> __device__ __forceinline__ E emax(const E& a, const E& b) { return lt(b, a) ? a : b; }
+template <> __device__ __forceinline__ E32 emin(const E32& a, const E32& b) { return {a.k < b.k ? a.k : b.k}; }
+template <> __device__ __forceinline__ E32 emax(const E32& a, const E32& b) { return {a.k < b.k ? b.k : a.k}; }
+
+// compare-exchange: a <- min, b <- max
+template <class E> __device__ __forceinline__ void cswap(E& a, E& b) {
+  E lo = emin(a, b), hi = emax(a, b);
+  a = lo;
+  b = hi;
+}
+
+template <class E> __device__ __forceinline__ E sel(bool c, const E& a, const E& b) { return c ? a : b; }
+
+template <class E> struct Limits;
+template <> struct Limits<E32> {
+  __device__ static E32 lo() { return {0u}; }
+  __device__ static E32 hi() { return {0xFFFFFFFFu}; }
+};
+template <> struct Limits<E64> {
+  __device__ static E64 lo() { return {0ull}; }
+  __device__ static E64 hi() { return {~0ull}; }
+};
+template <> struct Limits<P96> {
+  __device__ static P96 lo() { return {0ull, 0u}; }
+  __device__ static P96 hi() { return {~0ull, 0xFFFFFFFFu}; }
+};
+
+// 32-bit word w of an element (word planes in LDS, cross-lane moves)
+__device__ __forceinline__ uint32_t word(const E32& e, int) { return e.k; }
+__device__ __forceinline__ uint32_t word(const E64& e, int w) { return w ? (uint32_t)(e.k >> 32) : (uint32_t)e.k; }
+__device__ __forceinline__ uint32_t word(const P96& e, int w) {
+  return w == 2 ? e.v : (w ? (uint32_t)(e.k >> 32) : (uint32_t)e.k);
+}
+template <class E> __device__ __forceinline__ E from_words(const uint32_t* w);
+template <> __device__ __forceinline__ E32 from_words<E32>(const uint32_t* w) { return {w[0]}; }
+template <> __device__ __forceinline__ E64 from_words<E64>(const uint32_t* w) {
+  return {(uint64_t)w[0] | ((uint64_t)w[1] << 32)};
+}
+template <> __device__ __forceinline__ P96 from_words<P96>(const uint32_t* w) {
+  return {(uint64_t)w[0] | ((uint64_t)w[1] << 32), w[2]};
+}
+
+template <int M, class E>
+__device__ __forceinline__ E xlane(const E& e) {
+  uint32_t w[E::W];
+#pragma unroll
+  for (int i = 0; i < E::W; ++i) w[i] = xor_lane<M>(word(e, i));
+  return from_words<E>(w);
+}
+
+// ------------------------------------------------------------ dtype maps
+template <int DT> struct DType;
+template <> struct DType<DT_I32> {
+  using E = E32; using Word = uint32_t;
+  __device__ static uint32_t in(uint32_t x) { return x ^ 0x80000000u; }
+  __device__ static uint32_t out(uint32_t u) { return u ^ 0x80000000u; }
+};
+template <> struct DType<DT_U32> {
+  using E = E32; using Word = uint32_t;
+  __device__ static uint32_t in(uint32_t x) { return x; }
+  __device__ static uint32_t out(uint32_t u) { return u; }
+};
+template <> struct DType<DT_F32> {
+  using E = E32; using Word = uint32_t;
+  __device__ static uint32_t in(uint32_t x) { return x ^ ((uint32_t)((int32_t)x >> 31) | 0x80000000u); }
+  __device__ static uint32_t out(uint32_t u) { return u ^ ((uint32_t)((int32_t)~u >> 31) | 0x80000000u); }
+};
+template <> struct DType<DT_I64> {
+  using E = E64; using Word = uint64_t;
+  __device__ static uint64_t in(uint64_t x) { return x ^ 0x8000000000000000ull; }
+  __device__ static uint64_t out(uint64_t u) { return u ^ 0x8000000000000000ull; }
+};
+template <> struct DType<DT_U64> {
+  using E = E64; using Word = uint64_t;
+  __device__ static uint64_t in(uint64_t x) { return x; }
+  __device__ static uint64_t out(uint64_t u) { return u; }
+};
+template <> struct DType<DT_F64> {
+  using E = E64; using Word = uint64_t;
+  __device__ static uint64_t in(uint64_t x) { return x ^ ((uint64_t)((int64_t)x >> 63) | 0x8000000000000000ull); }
+  __device__ static uint64_t out(uint64_t u) { return u ^ ((uint64_t)((int64_t)~u >> 63) | 0x8000000000000000ull); }
+};
+template <> struct DType<DT_PAIR> {
+  using E = P96; using Word = uint64_t;
+  __device__ static uint64_t in(uint64_t x) { return x; }
+  __device__ static uint64_t out(uint64_t u) { return u; }
+};
+
+// Global element access. keys: Word array; vals: u32 array (pairs only).
+template <int DT, bool CONV>
+__device__ __forceinline__ typename DType<DT>::E gload(const void* keys, const uint32_t* vals, int64_t i) {
+  using D = DType<DT>;
+  using E = typename D::E;
+  using Wd = typename D::Word;
+  Wd x = static_cast<const Wd*>(keys)[i];
+  if (CONV) x = D::in(x);
+  E e;
+  if constexpr (E::HAS_VAL) {
+    e.k = x;
+    e.v = vals[i];
+  } else {
+    e.k = x;
+  }
+  return e;
+}
+template <int DT, bool CONV>
+__device__ __forceinline__ void gstore(void* keys, uint32_t* vals, int64_t i, const typename DType<DT>::E& e) {
+  using D = DType<DT>;
+  using E = typename D::E;
+  using Wd = typename D::Word;
+  Wd x = (Wd)e.k;
+  if (CONV) x = D::out(x);
+  static_cast<Wd*>(keys)[i] = x;
+  if constexpr (E::HAS_VAL) vals[i] = e.v;
+}
+
+}  // namespace gmms

@@ -1,0 +1,389 @@
+// One merge round: partition (findPartitions) + K-way minBlockHeap merge
+// (multimergeLevel / multimergePipeline).
+//
+// Reference: src/multimergesort.hxx:87-135 (round loop), :155-209
+// (multimergeLevel), src/warp-partition.hxx:179-366,487-532 (findPartitions,
+// single_pivot_partition), src/io-merge-gen.hxx:272-355 (buildHeap,
+// multimergePipeline).
+//
+// A "task" is K consecutive sorted lists of length L (the last task may be
+// ragged). A task of T elements is shared by m mergers; merger j writes output
+// ranks [j*T/m, (j+1)*T/m) of the task (the reference's target
+// warp-partition.hxx:234).
+//
+// Partition. Merger j needs a starting point. Instead of an exact co-rank at
+// its target rank r, k_partition finds a *consistent cut*: per-list offsets
+// c_k = number of elements of list k ordered before a pivot element y under
+// the stable order (key, list, position). Every element before the cut
+// precedes every element after it, so the merge of the lists from the cut on
+// is exactly the task's sorted sequence from rank R = sum(c_k). The search
+// keeps two consistent cuts lo (R_lo <= r) and hi (R_hi > r) and narrows them
+// with pivots drawn from the lists' windows -- interpolation steps alternated
+// with weighted-median steps (each of which removes >= 1/4 of the window) --
+// until r - R_lo <= slack. The merger then discards the first r - R_lo
+// outputs. Each pivot needs one windowed binary search per list, so the cost
+// is a few binary searches per list per merger; no list's pivot is ever
+// written by an idle merger (the reference's race, warp-partition.hxx:522-531).
+//
+// Merge. A minBlockHeap of 2K-1 nodes x 64 elements (one element per lane)
+// lives in LDS, word planes of 32 bits; every node's elements are <= all
+// elements of its children, so the root holds the 64 smallest remaining
+// elements. One wavefront per heap level (PIPELINE): each iteration the root
+// is emitted, every level-wave merges the two children of its node on the
+// root-to-leaf path (bitonic merge of 2x64 across lanes), keeps the lower 64
+// and sends the upper 64 to the sibling of the next path node, and the
+// emptied leaf is refilled from a per-list ring of prefetched blocks that
+// gfx950's global_load_lds DMA writes straight into LDS. Internal nodes start
+// as minimum-element blocks, so building the heap is just K-1 extra
+// iterations whose emissions are discarded; termination is by count, never by
+// a sentinel value (the reference stops at the first MAXVAL,
+// io-merge-gen.hxx:324, and truncates real INT_MAX keys).
+#pragma once
+#include "gmms_elem.hxx"
+
+namespace gmms {
+
+constexpr int MAX_K = 64;
+constexpr int MAX_PL = 6;
+
+struct RoundArgs {
+  const void* in_keys;
+  const uint32_t* in_vals;
+  void* out_keys;
+  uint32_t* out_vals;
+  int64_t n;      // elements in the array
+  int64_t L;      // list length this round
+  int64_t tasks;  // ceil(n / (K*L))
+  int K;          // lists per task (<= 64)
+  int PL;         // log2(K)
+  int m;          // mergers per task
+  int D;          // ring depth (blocks prefetched per list)
+  int64_t slack;  // partition tolerance (elements discarded at most)
+  int64_t* cuts;  // [(task*m + j) * K + k]
+  int64_t* skips; // [task*m + j]
+};
+
+__device__ __forceinline__ int64_t task_size(const RoundArgs& a, int64_t task) {
+  const int64_t tb = task * (int64_t)a.K * a.L;
+  const int64_t rest = a.n - tb;
+  const int64_t full = (int64_t)a.K * a.L;
+  return rest < full ? rest : full;
+}
+__device__ __forceinline__ int64_t list_len(const RoundArgs& a, int64_t T, int k) {
+  int64_t l = T - (int64_t)k * a.L;
+  if (l < 0) l = 0;
+  return l < a.L ? l : a.L;
+}
+
+template <int DT>
+__device__ __forceinline__ typename DType<DT>::E ld_int(const RoundArgs& a, int64_t i) {
+  return gload<DT, false>(a.in_keys, a.in_vals, i);
+}
+
+__device__ __forceinline__ int64_t wave_sum64(int64_t x) {
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) x += __shfl_xor(x, d, WAVE);
+  return x;
+}
+
+// -------------------------------------------------------------- partition
+constexpr int PART_WAVES = 4;
+
+template <int DT>
+__global__ __launch_bounds__(PART_WAVES * WAVE) void k_partition(RoundArgs a) {
+  using E = typename DType<DT>::E;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int64_t wid = (int64_t)blockIdx.x * PART_WAVES + threadIdx.x / WAVE;
+  const int64_t per_task = a.m - 1;
+  if (wid >= a.tasks * per_task) return;
+  const int64_t task = wid / per_task;
+  const int j = 1 + (int)(wid % per_task);
+  const int64_t T = task_size(a, task);
+  const int64_t r = (int64_t)j * T / a.m;
+  const bool inl = lane < a.K;
+  const int64_t nk = inl ? list_len(a, T, lane) : 0;
+  const int64_t lbase = task * (int64_t)a.K * a.L + (int64_t)lane * a.L;
+
+  int64_t lo = 0, hi = nk, Rlo = 0, Rhi = T;
+  for (int it = 0; it < 256 && r - Rlo > a.slack; ++it) {
+    const int64_t Wt = Rhi - Rlo;  // == sum(hi - lo)
+    const int64_t w = hi - lo;
+    const bool act = w > 0;
+    // interpolation on even steps (target r), weighted median on odd steps
+    const bool interp = (it & 1) == 0;
+    const int64_t tw = interp ? (r - Rlo) : (Wt >> 1);  // in [0, Wt)
+    int64_t pos = lo;
+    if (act) {
+      int64_t off = interp ? (int64_t)((double)(r - Rlo) / (double)Wt * (double)w) : (w >> 1);
+      if (off < 0) off = 0;
+      pos = lo + (off < w ? off : w - 1);
+    }
+    const E x = act ? ld_int<DT>(a, lbase + pos) : Limits<E>::hi();
+    // rank of this candidate among active candidates, order (element, lane)
+    int rank = 0;
+    for (int i = 0; i < a.K; ++i) {
+      const bool ai = __shfl((int)act, i, WAVE) != 0;
+      uint32_t wi[E::W];
+#pragma unroll
+      for (int q = 0; q < E::W; ++q) wi[q] = (uint32_t)__shfl((int)word(x, q), i, WAVE);
+      const E xi = from_words<E>(wi);
+      rank += (ai && (lt(xi, x) || (!lt(x, xi) && i < lane))) ? 1 : 0;
+    }
+    // inclusive weight prefix in sorted order
+    int64_t pre = 0;
+    for (int i = 0; i < a.K; ++i) {
+      const int ri = __shfl(rank, i, WAVE);
+      const int64_t wi = __shfl(act ? w : (int64_t)0, i, WAVE);
+      const bool ai = __shfl((int)act, i, WAVE) != 0;
+      if (ai && ri <= rank) pre += wi;
+    }
+    const bool is_piv = act && (pre - w) <= tw && tw < pre;
+    const unsigned long long bal = __ballot(is_piv);
+    const int kp = __ffsll((long long)bal) - 1;
+    uint32_t yw[E::W];
+#pragma unroll
+    for (int q = 0; q < E::W; ++q) yw[q] = (uint32_t)__shfl((int)word(x, q), kp, WAVE);
+    const E y = from_words<E>(yw);
+    const int64_t ppos = __shfl(pos, kp, WAVE);
+    // count of list elements ordered before y, inside [lo, hi]
+    int64_t c;
+    if (lane == kp) {
+      c = ppos;
+    } else {
+      int64_t l = lo, h = hi;  // answer in [l, h]
+      const bool upper = lane < kp;  // equal keys of lower lists come first
+      while (l < h) {
+        const int64_t mid = l + ((h - l) >> 1);
+        const E v = ld_int<DT>(a, lbase + mid);
+        const bool before = upper ? !lt(y, v) : lt(v, y);
+        if (before) l = mid + 1; else h = mid;
+      }
+      c = l;
+    }
+    if (!inl) c = 0;
+    const int64_t R = wave_sum64(c);
+    if (R <= r) {
+      lo = c;
+      Rlo = R;
+      if (R < r && lane == kp) lo = c + 1;  // the pivot itself is taken too
+      if (R < r) Rlo = R + 1;
+    } else {
+      hi = c;
+      Rhi = R;
+    }
+  }
+  const int64_t b = task * a.m + j;
+  if (inl) a.cuts[b * a.K + lane] = lo;
+  if (lane == 0) a.skips[b] = r - Rlo;
+}
+
+// ------------------------------------------------------------------ merge
+// s_waitcnt vmcnt(n) with n a run-time value: clamp and dispatch.
+template <int N>
+__device__ __forceinline__ void wait_vm_ge(int n) {
+  if constexpr (N == 0) {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt/lgkmcnt untouched
+  } else {
+    if (n >= N) {
+      // gfx9 encoding: vmcnt[3:0] bits 3:0, vmcnt[5:4] bits 15:14
+      __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | 0x0F70);
+    } else {
+      wait_vm_ge<N - 1>(n);
+    }
+  }
+}
+__device__ __forceinline__ void wait_vm(int n) { wait_vm_ge<31>(n); }
+
+template <class E>
+__device__ __forceinline__ void bitonic_clean64(E& v, int lane) {
+  {
+    E p = xlane<32>(v);
+    v = (lane & 32) ? emax(v, p) : emin(v, p);
+  }
+  {
+    E p = xlane<16>(v);
+    v = (lane & 16) ? emax(v, p) : emin(v, p);
+  }
+  {
+    E p = xlane<8>(v);
+    v = (lane & 8) ? emax(v, p) : emin(v, p);
+  }
+  {
+    E p = xlane<4>(v);
+    v = (lane & 4) ? emax(v, p) : emin(v, p);
+  }
+  {
+    E p = xlane<2>(v);
+    v = (lane & 2) ? emax(v, p) : emin(v, p);
+  }
+  {
+    E p = xlane<1>(v);
+    v = (lane & 1) ? emax(v, p) : emin(v, p);
+  }
+}
+
+template <class E>
+__device__ __forceinline__ E lds_get(const uint32_t* plane0, int stride, int idx) {
+  uint32_t w[E::W];
+#pragma unroll
+  for (int q = 0; q < E::W; ++q) w[q] = plane0[q * stride + idx];
+  return from_words<E>(w);
+}
+template <class E>
+__device__ __forceinline__ void lds_put(uint32_t* plane0, int stride, int idx, const E& e) {
+#pragma unroll
+  for (int q = 0; q < E::W; ++q) plane0[q * stride + idx] = word(e, q);
+}
+
+// LDS words needed by one merger
+__host__ __device__ inline size_t merge_lds_words(int K, int W, int D) {
+  const size_t nodes = (size_t)(2 * K - 1);
+  return nodes * W * WAVE + nodes * W + (size_t)K * D * W * WAVE + (size_t)K * D;
+}
+
+template <int DT, bool CONV_OUT>
+__global__ __launch_bounds__(MAX_PL * WAVE) void k_merge(RoundArgs a) {
+  using E = typename DType<DT>::E;
+  constexpr int W = E::W;
+  extern __shared__ uint32_t lds[];
+  const int K = a.K, PL = a.PL, D = a.D;
+  const int NN = 2 * K - 1;
+  uint32_t* nodes = lds;                            // [node][plane][64]
+  uint32_t* nmax = nodes + (size_t)NN * W * WAVE;   // [plane][node]
+  uint32_t* ring = nmax + (size_t)NN * W;           // [k][d][plane][64]
+  int* issued = (int*)(ring + (size_t)K * D * W * WAVE);  // [k*D + d]
+
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wave = threadIdx.x / WAVE;
+  const int64_t task = blockIdx.x / a.m;
+  const int j = blockIdx.x % a.m;
+  const int64_t T = task_size(a, task);
+  const int64_t tbase = task * (int64_t)K * a.L;
+  const int64_t r0 = (int64_t)j * T / a.m;
+  const int64_t r1 = (int64_t)(j + 1) * T / a.m;
+  const int64_t count = r1 - r0;
+  const int64_t skip = j == 0 ? 0 : a.skips[blockIdx.x];
+
+  // internal nodes start as minimum blocks
+  for (int i = threadIdx.x; i < (K - 1) * W * WAVE; i += blockDim.x) nodes[i] = 0u;
+  for (int i = threadIdx.x; i < NN * W; i += blockDim.x) nmax[i] = 0u;
+  __syncthreads();
+
+  const bool bottom = wave == PL - 1;
+  int64_t lstart = 0, lrem = 0;  // lane k: list k from its cut
+  int cons = 1;                  // lane k: next block index to consume
+  int G = 0;                     // DMA instructions issued by the bottom wave
+
+  auto dma_block = [&](int k, int64_t st, int64_t rem, int blk, uint32_t* dst) {
+    const int64_t idx = (int64_t)blk * WAVE + lane;
+    if (idx < rem) {
+      const int64_t gi = st + idx;
+      if constexpr (W == 1) {
+        __builtin_amdgcn_global_load_lds((const void*)((const uint32_t*)a.in_keys + gi),
+                                         (__attribute__((address_space(3))) void*)dst, 4, 0, 0);
+      } else {
+        const uint32_t* kw = (const uint32_t*)a.in_keys + 2 * gi;
+        __builtin_amdgcn_global_load_lds((const void*)kw, (__attribute__((address_space(3))) void*)dst, 4, 0, 0);
+        __builtin_amdgcn_global_load_lds((const void*)(kw + 1),
+                                         (__attribute__((address_space(3))) void*)(dst + WAVE), 4, 0, 0);
+        if constexpr (W == 3) {
+          __builtin_amdgcn_global_load_lds((const void*)(a.in_vals + gi),
+                                           (__attribute__((address_space(3))) void*)(dst + 2 * WAVE), 4, 0, 0);
+        }
+      }
+    }
+    (void)k;
+  };
+
+  if (bottom) {
+    if (lane < K) {
+      const int64_t nk = list_len(a, T, lane);
+      const int64_t ck = j == 0 ? 0 : a.cuts[(int64_t)blockIdx.x * K + lane];
+      lstart = tbase + (int64_t)lane * a.L + ck;
+      lrem = nk - ck;
+    }
+    for (int k = 0; k < K; ++k) {
+      const int64_t st = __shfl(lstart, k, WAVE);
+      const int64_t rem = __shfl(lrem, k, WAVE);
+      uint32_t* leaf = nodes + (size_t)(K - 1 + k) * W * WAVE;
+      if ((int64_t)lane >= rem) lds_put<E>(leaf, WAVE, lane, Limits<E>::hi());
+      dma_block(k, st, rem, 0, leaf);
+      for (int d = 0; d < D; ++d) {
+        dma_block(k, st, rem, d + 1, ring + ((size_t)(k * D + d) * W) * WAVE);
+        G += W;
+        if (lane == 0) issued[k * D + d] = G;
+      }
+    }
+    wait_vm(0);
+    __builtin_amdgcn_s_waitcnt(0);
+    if (lane < K) {
+      const E mx = lrem >= WAVE ? lds_get<E>(nodes + (size_t)(K - 1 + lane) * W * WAVE + (WAVE - 1), WAVE, 0)
+                                : Limits<E>::hi();
+      lds_put<E>(nmax, NN, K - 1 + lane, mx);
+    }
+  }
+  __syncthreads();
+
+  E root = Limits<E>::lo();
+  const int64_t pre = (int64_t)WAVE * (K - 1) + skip;  // discarded emissions
+  const int64_t iters = (pre + count + WAVE - 1) / WAVE;
+  const int64_t obase = tbase + r0;
+  for (int64_t it = 0; it < iters; ++it) {
+    if (wave == 0) {
+      const int64_t o = it * WAVE + lane - pre;
+      if (o >= 0 && o < count) gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + o, root);
+    }
+    if (it == iters - 1) break;
+    // root-to-leaf path: the child with the smaller maximum is emptied
+    int p = 0, pn = 0;
+    for (int l = 0; l <= wave; ++l) {
+      const int c1 = 2 * p + 1;
+      const E m1 = lds_get<E>(nmax, NN, c1);
+      const E m2 = lds_get<E>(nmax, NN, c1 + 1);
+      const int q = le(m1, m2) ? c1 : c1 + 1;
+      if (l < wave) p = q; else pn = q;
+    }
+    const int c1 = 2 * p + 1;
+    const E va = lds_get<E>(nodes + (size_t)c1 * W * WAVE, WAVE, lane);
+    const E vb = lds_get<E>(nodes + (size_t)(c1 + 1) * W * WAVE, WAVE, WAVE - 1 - lane);
+    __syncthreads();
+    E lo = emin(va, vb), hi = emax(va, vb);
+    bitonic_clean64(lo, lane);
+    bitonic_clean64(hi, lane);
+    const int sib = (pn == c1) ? c1 + 1 : c1;
+    lds_put<E>(nodes + (size_t)sib * W * WAVE, WAVE, lane, hi);
+    if (lane == WAVE - 1) lds_put<E>(nmax, NN, sib, hi);
+    if (wave == 0) {
+      root = lo;
+    } else {
+      lds_put<E>(nodes + (size_t)p * W * WAVE, WAVE, lane, lo);
+      if (lane == WAVE - 1) lds_put<E>(nmax, NN, p, lo);
+    }
+    if (bottom) {
+      // refill leaf pn from the ring
+      const int k = pn - (K - 1);
+      const int blk = __shfl(cons, k, WAVE);
+      const int64_t st = __shfl(lstart, k, WAVE);
+      const int64_t rem = __shfl(lrem, k, WAVE);
+      const int d = (blk - 1) % D;
+      const int isd = issued[k * D + d];
+      if (PL == 1) wait_vm(0); else wait_vm(G - isd);
+      uint32_t* slot = ring + ((size_t)(k * D + d) * W) * WAVE;
+      const int64_t idx = (int64_t)blk * WAVE + lane;
+      E v = idx < rem ? lds_get<E>(slot, WAVE, lane) : Limits<E>::hi();
+      uint32_t* leaf = nodes + (size_t)pn * W * WAVE;
+      lds_put<E>(leaf, WAVE, lane, v);
+      if (lane == WAVE - 1) lds_put<E>(nmax, NN, pn, v);
+      if (lane == k) cons = blk + 1;
+      if ((int64_t)(blk + D) * WAVE < rem) {
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): slot read done before DMA reuses it
+        dma_block(k, st, rem, blk + D, slot);
+        G += W;
+        if (lane == 0) issued[k * D + d] = G;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace gmms

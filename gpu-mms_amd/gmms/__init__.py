@@ -1,0 +1,311 @@
+"""Python host side of the MI355X-native multiway mergesort.
+
+A thin ctypes layer over the C ABI of ``libgmms.so`` (declared in
+``include/gpu_mms/gmms.h``). PyTorch is used only for device memory and
+streams. The product path is the HIP library: importing this module on a
+machine where ``libgmms.so`` is missing raises, there is no CPU fallback.
+
+Mirrors the reference's operator interface (algoparc/GPU-MMS):
+
+* :func:`multimergesort` -- ``multimergesort<T,cmp>(input, output, host_array,
+  threadBlocks, N)`` (src/multimergesort.hxx:56-141, README.md:16-25): sorts
+  ``input`` using ``output`` as the second buffer and returns whichever holds
+  the result.
+* :func:`sort` -- the same sort with a caller-visible workspace and stream,
+  returning fresh sorted tensors (keys, or keys and values for pairs).
+* :func:`squaresort`, :func:`merge_round`, :func:`find_partitions`,
+  :func:`pad` -- the reference's kernels squareSort, multimergeLevel,
+  findPartitions and pad as single stages.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+from dataclasses import dataclass
+from typing import Optional, Tuple
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgmms.so")
+HEADER_PATH = os.path.normpath(os.path.join(_HERE, "..", "..", "include", "gpu_mms", "gmms.h"))
+
+# dtype codes (gmms.h: gmms_dtype)
+INT32, UINT32, FLOAT32, INT64, UINT64, FLOAT64, PAIR_U64_U32 = range(7)
+DTYPE_NAMES = {
+    INT32: "int32", UINT32: "uint32", FLOAT32: "float32", INT64: "int64",
+    UINT64: "uint64", FLOAT64: "float64", PAIR_U64_U32: "pair_u64_u32",
+}
+
+SUCCESS = 0
+ERROR_INVALID_VALUE = 1
+ERROR_UNSUPPORTED = 2
+ERROR_WORKSPACE = 3
+ERROR_HIP = 4
+ERROR_OUT_OF_MEMORY = 5
+ERROR_BUDGET = 6
+
+TILE = 1024  # base-case tile (the reference's M, src/params.h:36)
+
+
+class GmmsError(RuntimeError):
+    def __init__(self, status: int, where: str):
+        self.status = status
+        super().__init__(f"{where}: {status_string(status)} (status {status})")
+
+
+class Params(ctypes.Structure):
+    """gmms_params (gmms.h); replaces src/params.h:18-38 (K, PIPELINE, BLOCKS)."""
+
+    _fields_ = [
+        ("K", ctypes.c_int32),
+        ("pipeline", ctypes.c_int32),
+        ("mergers", ctypes.c_int32),
+        ("prefetch", ctypes.c_int32),
+        ("slack", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
+    ]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libgmms.so once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libgmms.so not built at {LIB_PATH}; run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i64, i32, sz = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_size_t
+    P = ctypes.POINTER(Params)
+    sig = {
+        "gmms_version": (ctypes.c_char_p, []),
+        "gmms_status_string": (ctypes.c_char_p, [i32]),
+        "gmms_get_default_params": (i32, [P]),
+        "gmms_element_size": (i32, [i32, ctypes.POINTER(sz), ctypes.POINTER(sz)]),
+        "gmms_num_rounds": (i32, [i64, i32, ctypes.POINTER(i32)]),
+        "gmms_workspace_size": (i32, [i32, i64, P, ctypes.POINTER(sz)]),
+        "gmms_round_workspace_size": (i32, [i32, i64, i64, P, ctypes.POINTER(sz)]),
+        "gmms_sort": (i32, [i32, vp, vp, vp, vp, i64, P, vp, sz, vp, ctypes.POINTER(i32)]),
+        "gmms_multimergesort": (i32, [i32, vp, vp, vp, i32, i64, ctypes.POINTER(vp)]),
+        "gmms_squaresort": (i32, [i32, vp, vp, i64, i32, i32, vp]),
+        "gmms_merge_round": (i32, [i32, vp, vp, vp, vp, i64, i64, P, i32, vp, sz, vp]),
+        "gmms_find_partitions": (i32, [i32, vp, vp, i64, i64, P, vp, vp, i64, ctypes.POINTER(i32), vp]),
+        "gmms_pad": (i32, [i32, vp, vp, i64, vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def declared_symbols(header: str = HEADER_PATH) -> list:
+    """Every function name declared in include/gpu_mms/*.h (for export checks)."""
+    text = open(header).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gmms_[a-z0-9_]+)\s*\(", text)))
+
+
+def status_string(status: int) -> str:
+    return lib().gmms_status_string(status).decode()
+
+
+def version() -> str:
+    return lib().gmms_version().decode()
+
+
+def _check(status: int, where: str) -> None:
+    if status != SUCCESS:
+        raise GmmsError(status, where)
+
+
+def default_params() -> Params:
+    p = Params()
+    _check(lib().gmms_get_default_params(ctypes.byref(p)), "gmms_get_default_params")
+    return p
+
+
+def make_params(K: int = 0, mergers: int = 0, prefetch: int = 0, slack: int = 0) -> Params:
+    p = default_params()
+    if K:
+        p.K = K
+    if mergers:
+        p.mergers = mergers
+    if prefetch:
+        p.prefetch = prefetch
+    if slack:
+        p.slack = slack
+    return p
+
+
+def num_rounds(n: int, K: int = 64) -> int:
+    r = ctypes.c_int()
+    _check(lib().gmms_num_rounds(n, K, ctypes.byref(r)), "gmms_num_rounds")
+    return r.value
+
+
+def workspace_size(dtype: int, n: int, params: Optional[Params] = None) -> int:
+    b = ctypes.c_size_t()
+    _check(lib().gmms_workspace_size(dtype, n, _pp(params), ctypes.byref(b)), "gmms_workspace_size")
+    return b.value
+
+
+def _pp(params: Optional[Params]):
+    return ctypes.byref(params) if params is not None else None
+
+
+# ---------------------------------------------------------------- torch side
+def _torch():
+    import torch  # noqa: WPS433 -- torch is only plumbing for device memory
+
+    return torch
+
+
+def dtype_of(t) -> int:
+    torch = _torch()
+    m = {
+        torch.int32: INT32, torch.uint32: UINT32, torch.float32: FLOAT32,
+        torch.int64: INT64, torch.uint64: UINT64, torch.float64: FLOAT64,
+    }
+    if t.dtype not in m:
+        raise GmmsError(ERROR_INVALID_VALUE, f"unsupported tensor dtype {t.dtype}")
+    return m[t.dtype]
+
+
+def _ptr(t) -> Optional[int]:
+    return None if t is None else t.data_ptr()
+
+
+def _stream_ptr(stream) -> Optional[int]:
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def _check_dev(*ts) -> None:
+    for t in ts:
+        if t is not None and (not t.is_cuda or not t.is_contiguous()):
+            raise GmmsError(ERROR_INVALID_VALUE, "tensors must be contiguous device tensors")
+
+
+@dataclass
+class Workspace:
+    """Caller-owned device workspace (partition cuts) reused across calls."""
+
+    tensor: object = None
+
+    def ensure(self, nbytes: int, device) -> Tuple[Optional[int], int]:
+        torch = _torch()
+        if nbytes == 0:
+            return None, 0
+        if self.tensor is None or self.tensor.numel() < nbytes or self.tensor.device != device:
+            self.tensor = torch.empty(nbytes, dtype=torch.uint8, device=device)
+        return self.tensor.data_ptr(), self.tensor.numel()
+
+
+_default_ws = Workspace()
+
+
+def sort_inplace(keys, keys_alt, values=None, values_alt=None, *, dtype: Optional[int] = None,
+                 params: Optional[Params] = None, workspace: Optional[Workspace] = None,
+                 stream=None) -> int:
+    """gmms_sort on preallocated buffers; returns 0 if the result is in
+    keys/values, 1 if it is in keys_alt/values_alt. Asynchronous."""
+    _check_dev(keys, keys_alt, values, values_alt)
+    dt = dtype if dtype is not None else (PAIR_U64_U32 if values is not None else dtype_of(keys))
+    n = keys.numel()
+    ws = workspace or _default_ws
+    nbytes = workspace_size(dt, n, params)
+    wp, wb = ws.ensure(nbytes, keys.device)
+    alt = ctypes.c_int(0)
+    st = lib().gmms_sort(dt, _ptr(keys), _ptr(keys_alt), _ptr(values), _ptr(values_alt), n, _pp(params),
+                         wp, wb, _stream_ptr(stream), ctypes.byref(alt))
+    _check(st, "gmms_sort")
+    return alt.value
+
+
+def sort(keys, values=None, *, params: Optional[Params] = None, workspace: Optional[Workspace] = None,
+         stream=None):
+    """Sort a device tensor (copy). Pairs: keys uint64/int64 viewed as uint64
+    and values uint32/int32 viewed as uint32, ordered by (key, value).
+    Returns the sorted keys, or (keys, values)."""
+    torch = _torch()
+    k = keys.contiguous().clone()
+    ka = torch.empty_like(k)
+    if values is None:
+        alt = sort_inplace(k, ka, dtype=dtype_of(k), params=params, workspace=workspace, stream=stream)
+        return ka if alt else k
+    v = values.contiguous().clone()
+    va = torch.empty_like(v)
+    if k.element_size() != 8 or v.element_size() != 4 or k.numel() != v.numel():
+        raise GmmsError(ERROR_INVALID_VALUE, "pairs need 64-bit keys and 32-bit values of equal length")
+    alt = sort_inplace(k, ka, v, va, dtype=PAIR_U64_U32, params=params, workspace=workspace, stream=stream)
+    return (ka, va) if alt else (k, v)
+
+
+def multimergesort(input, output, host_array=None, thread_blocks: int = 0, n: Optional[int] = None,
+                   dtype: Optional[int] = None):
+    """The reference's drop-in call: returns `input` or `output`, whichever
+    holds the sorted data (synchronous, allocates its own workspace)."""
+    _check_dev(input, output)
+    dt = dtype if dtype is not None else dtype_of(input)
+    n = input.numel() if n is None else n
+    res = ctypes.c_void_p()
+    st = lib().gmms_multimergesort(dt, _ptr(input), _ptr(output), None, thread_blocks, n, ctypes.byref(res))
+    _check(st, "gmms_multimergesort")
+    return output if res.value == output.data_ptr() else input
+
+
+def squaresort(keys, values=None, *, dtype: Optional[int] = None, to_internal: bool = True,
+               to_external: bool = True, stream=None) -> None:
+    """squareSort: sort every 1024-element tile in place."""
+    _check_dev(keys, values)
+    dt = dtype if dtype is not None else (PAIR_U64_U32 if values is not None else dtype_of(keys))
+    st = lib().gmms_squaresort(dt, _ptr(keys), _ptr(values), keys.numel(), int(to_internal), int(to_external),
+                               _stream_ptr(stream))
+    _check(st, "gmms_squaresort")
+
+
+def merge_round(in_keys, out_keys, list_len: int, in_values=None, out_values=None, *,
+                dtype: Optional[int] = None, params: Optional[Params] = None, to_external: bool = False,
+                workspace: Optional[Workspace] = None, stream=None) -> None:
+    """One merge round (findPartitions + multimergeLevel) on internal-order data."""
+    _check_dev(in_keys, out_keys, in_values, out_values)
+    dt = dtype if dtype is not None else (PAIR_U64_U32 if in_values is not None else dtype_of(in_keys))
+    n = in_keys.numel()
+    ws = workspace or _default_ws
+    b = ctypes.c_size_t()
+    _check(lib().gmms_round_workspace_size(dt, n, list_len, _pp(params), ctypes.byref(b)),
+           "gmms_round_workspace_size")
+    nbytes = b.value
+    wp, wb = ws.ensure(nbytes, in_keys.device)
+    st = lib().gmms_merge_round(dt, _ptr(in_keys), _ptr(in_values), _ptr(out_keys), _ptr(out_values), n,
+                                list_len, _pp(params), int(to_external), wp, wb, _stream_ptr(stream))
+    _check(st, "gmms_merge_round")
+
+
+def find_partitions(keys, list_len: int, values=None, *, dtype: Optional[int] = None,
+                    params: Optional[Params] = None, cap_mergers: int = 1 << 16, stream=None):
+    """findPartitions: returns (cuts[mergers, K], skips[mergers], m) on device."""
+    torch = _torch()
+    _check_dev(keys, values)
+    dt = dtype if dtype is not None else (PAIR_U64_U32 if values is not None else dtype_of(keys))
+    p = params if params is not None else default_params()
+    K = p.K or 64
+    cuts = torch.zeros(cap_mergers * K, dtype=torch.int64, device=keys.device)
+    skips = torch.zeros(cap_mergers, dtype=torch.int64, device=keys.device)
+    m = ctypes.c_int(0)
+    st = lib().gmms_find_partitions(dt, _ptr(keys), _ptr(values), keys.numel(), list_len, ctypes.byref(p),
+                                    _ptr(cuts), _ptr(skips), cap_mergers, ctypes.byref(m), _stream_ptr(stream))
+    _check(st, "gmms_find_partitions")
+    return cuts.view(cap_mergers, K), skips, m.value
+
+
+def pad(keys, count: int, values=None, *, dtype: Optional[int] = None, stream=None) -> None:
+    """pad: write `count` maximum elements at the start of keys (and values)."""
+    _check_dev(keys, values)
+    dt = dtype if dtype is not None else (PAIR_U64_U32 if values is not None else dtype_of(keys))
+    _check(lib().gmms_pad(dt, _ptr(keys), _ptr(values), count, _stream_ptr(stream)), "gmms_pad")

@@ -1,0 +1,149 @@
+/*
+ * gmms.h -- C ABI of the MI355X-native multiway mergesort (libgmms.so).
+ *
+ * Plain pointers and sizes only: device pointers are HIP device memory on the
+ * current device, `stream` is a hipStream_t passed as void* (NULL = the null
+ * stream). Every entry point returns a gmms_status; none prints or aborts.
+ *
+ * Each declaration names the reference interface it replaces
+ * (algoparc/GPU-MMS, paths relative to the reference repository root). The
+ * reference has no FFI of its own: its boundary is the header-only template
+ * multimergesort<T,f> (src/multimergesort.hxx:56-57, documented with a
+ * threadBlocks argument at README.md:16-25). These functions are what a
+ * foreign caller (ctypes, cgo, JNI) binds instead; the C++ template in
+ * multimergesort.hxx is built on top of them.
+ */
+#ifndef GPU_MMS_GMMS_H
+#define GPU_MMS_GMMS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. The reference returns nothing and only prints under DEBUG
+ * (src/multimergesort.hxx:102-108,129-134). */
+typedef enum gmms_status {
+  GMMS_SUCCESS = 0,
+  GMMS_ERROR_INVALID_VALUE = 1, /* bad pointer, size, dtype or parameter */
+  GMMS_ERROR_UNSUPPORTED = 2,   /* valid request this build does not run  */
+  GMMS_ERROR_WORKSPACE = 3,     /* workspace missing or too small         */
+  GMMS_ERROR_HIP = 4,           /* a HIP runtime call or launch failed    */
+  GMMS_ERROR_OUT_OF_MEMORY = 5, /* device allocation failed               */
+  GMMS_ERROR_BUDGET = 6         /* serving: request exceeds the HBM budget */
+} gmms_status;
+
+/* Element types with their built-in comparators. Replaces the compile-time
+ * DATATYPE / cmp<T> pair of src/cmp.hxx:21-46: integers order by '<',
+ * floats by IEEE total order (-NaN < -inf < ... < -0 < +0 < ... < +inf <
+ * +NaN), pairs by (key, value). */
+typedef enum gmms_dtype {
+  GMMS_INT32 = 0,
+  GMMS_UINT32 = 1,
+  GMMS_FLOAT32 = 2,
+  GMMS_INT64 = 3,
+  GMMS_UINT64 = 4,
+  GMMS_FLOAT64 = 5,
+  GMMS_PAIR_U64_U32 = 6 /* keys: uint64_t[n], values: uint32_t[n] */
+} gmms_dtype;
+
+/* Run-time knobs. Replaces the compile-time macros of src/params.h:18-38:
+ * K (branching factor, a power of two, 2..64), PIPELINE (one wavefront per
+ * heap level; 0 is accepted and means the same pipelined kernel), and
+ * BLOCKS (= threadBlocks, the merger count when a round has few tasks).
+ * Zero fields take the defaults of gmms_get_default_params. */
+typedef struct gmms_params {
+  int32_t K;         /* lists merged per round; 0: auto = the largest power
+                        of two <= 64 whose heap fits LDS (64 for 4- and
+                        8-byte keys, 32 for pairs)                           */
+  int32_t pipeline;  /* 1: one wave per heap level (the only kernel)          */
+  int32_t mergers;   /* target mergers per round when tasks are few; 0: auto */
+  int32_t prefetch;  /* blocks prefetched per list into LDS (1..8)           */
+  int32_t slack;     /* partition tolerance in elements; 0: auto             */
+  int32_t reserved[3];
+} gmms_params;
+
+/* Library version string, e.g. "gpu-mms-amd 0.1.0 gfx950". */
+const char* gmms_version(void);
+/* Human-readable name of a status code. */
+const char* gmms_status_string(int status);
+/* Fill *p with the defaults (K auto, PIPELINE on, auto mergers). */
+int gmms_get_default_params(gmms_params* p);
+/* Bytes per key and per value (0 for key-only dtypes) of a dtype. */
+int gmms_element_size(int dtype, size_t* key_bytes, size_t* value_bytes);
+/* Number of merge rounds for n elements: iterations of the reference's
+ * `for (listSize = M; listSize < N; listSize *= K)` (multimergesort.hxx:87). */
+int gmms_num_rounds(int64_t n, int K, int* rounds);
+
+/* Device workspace (partition cuts) that gmms_sort needs for n elements.
+ * The reference cudaMalloc's its pivot array inside every call
+ * (multimergesort.hxx:61-63,138); here the caller owns it so the timed
+ * path never allocates. */
+int gmms_workspace_size(int dtype, int64_t n, const gmms_params* p, size_t* bytes);
+/* Workspace for one gmms_merge_round over n elements in lists of list_len. */
+int gmms_round_workspace_size(int dtype, int64_t n, int64_t list_len, const gmms_params* p,
+                              size_t* bytes);
+
+/* Sort n elements. Replaces multimergesort<T,f>(input, output, h_data,
+ * threadBlocks, N) (src/multimergesort.hxx:56-141, README.md:16-25) with the
+ * same buffer contract: both key buffers (and both value buffers for pairs)
+ * are clobbered, the base case sorts `keys` in place and the rounds
+ * ping-pong; *result_in_alt is set to 0 if the sorted data ends in
+ * keys/vals and 1 if it ends in keys_alt/vals_alt (the reference returns
+ * that pointer, :140). vals/vals_alt must be NULL for key-only dtypes.
+ * Asynchronous on `stream`; no allocation, no host synchronisation. */
+int gmms_sort(int dtype, void* keys, void* keys_alt, void* vals, void* vals_alt,
+              int64_t n, const gmms_params* p, void* workspace,
+              size_t workspace_bytes, void* stream, int* result_in_alt);
+
+/* The reference's entry point with its own argument list
+ * (src/multimergesort.hxx:56-57 plus README.md's threadBlocks): sorts the
+ * device array `input` of n elements using `output` as the second buffer,
+ * allocates and frees its workspace itself and synchronises before it
+ * returns, like the reference; *result receives `input` or `output`,
+ * whichever holds the sorted data. host_array is accepted and unused, as in
+ * the reference (it is a debugging aid there). thread_blocks <= 0: auto. */
+int gmms_multimergesort(int dtype, void* input, void* output, void* host_array,
+                        int thread_blocks, int64_t n, void** result);
+
+/* ---- single stages, for tests and for callers that drive rounds ------- */
+
+/* squareSort (src/basecase/squareSort.hxx:62-131): sort every 1024-element
+ * tile of keys[0..n) in place, the ragged tail as its own tile. to_internal /
+ * to_external select the order-preserving bit maps on load / store (pass 1/1
+ * for a plain tile sort). */
+int gmms_squaresort(int dtype, void* keys, void* vals, int64_t n, int to_internal,
+                    int to_external, void* stream);
+
+/* One merge round (findPartitions + multimergeLevel,
+ * src/multimergesort.hxx:89-128): in holds sorted runs of list_len elements;
+ * out receives sorted runs of K*list_len. Elements must be in the internal
+ * order (as left by gmms_squaresort(..., to_internal=1, to_external=0));
+ * to_external=1 maps the output back to the user's bits. */
+int gmms_merge_round(int dtype, const void* in_keys, const void* in_vals,
+                     void* out_keys, void* out_vals, int64_t n, int64_t list_len,
+                     const gmms_params* p, int to_external, void* workspace,
+                     size_t workspace_bytes, void* stream);
+
+/* Partition only (findPartitions, src/warp-partition.hxx:487-532): for the
+ * round with lists of list_len, writes for every merger b = task*m + j the
+ * per-list cut offsets cuts[b*K + k] (consistent: all elements before the
+ * cut precede all after it under (key, list, position)) and skips[b] =
+ * target rank - rank of the cut, 0 <= skips[b] <= slack. j = 0 entries are
+ * written as zero. *mergers_per_task receives m. Internal order. */
+int gmms_find_partitions(int dtype, const void* keys, const void* vals, int64_t n,
+                         int64_t list_len, const gmms_params* p, int64_t* cuts,
+                         int64_t* skips, int64_t cap_mergers,
+                         int* mergers_per_task, void* stream);
+
+/* pad (src/pad.hxx:1-4): write `count` copies of the dtype's maximum
+ * element (all-ones internal bits) at data[0..count). The sort itself never
+ * needs padding (ragged tiles are padded in registers). */
+int gmms_pad(int dtype, void* keys, void* vals, int64_t count, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPU_MMS_GMMS_H */
