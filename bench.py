@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Benchmark: plain sort of N = 2^28 int32 keys, uniform over the full int32
+range, on one MI355X through libgmms.so; prints one JSON line.
+
+A step is one complete sort (base case + every merge round) of one resident
+2^28-key array; the timed region holds exactly --steps sorts back to back,
+each on its own pre-generated input (input i = input 0 XOR a per-step
+constant, so no step sorts sorted data), with no allocation and no copies.
+
+Reported:
+  value      keys sorted per second over the timed region (GKeys/s)
+  roofline   the dominant kernel (the K-way merge round): algorithmic bytes
+             2*N*4 per launch over its average launch time, measured with HIP
+             events recorded on the launch stream during the timed region;
+             peak = MI355X's published 8 TB/s. traffic = HBM bytes per merge
+             launch from rocprofv3 FETCH_SIZE/WRITE_SIZE (profiles/), if
+             recorded, else null.
+  cpu_baseline  std::sort (1 thread, on a 2^25-key sample) and libstdc++
+             parallel sort (all host threads, full input) of the same keys,
+             timed on this host; the parallel figure is the reported one.
+The GPU output of input 0 is compared bit-exact with the CPU sort.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gpu-mms_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+PUBLISHED_HBM_PEAK_GBPS = 8000.0
+METRIC = "GKeys/s + % HBM roofline at 2^28 int32; batched-segment vs per-array launch"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1 << 28)
+    ap.add_argument("--K", type=int, default=0, help="lists per merge round (0: auto)")
+    ap.add_argument("--mergers", type=int, default=0)
+    ap.add_argument("--prefetch", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def traffic_from_profiles():
+    """HBM bytes per merge launch measured by rocprofv3 (profiles/merge_traffic.json)."""
+    p = os.path.join(ROOT, "profiles", "merge_traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("hbm_bytes_per_merge_launch")
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design note "
+              "(see DESIGN.md), --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import gmms
+    from conftest import Oracle
+
+    dev = torch.device("cuda:0")
+    n = args.n
+    oracle = Oracle()
+    seed = 0x6D6D7301
+    host = oracle.gen_i32(seed, n, 32)  # (int32)(u64 >> 32): the full int32 range
+    base = torch.from_numpy(host).to(dev)
+    total = args.warmup + args.steps
+    rng = np.random.default_rng(seed)
+    consts = [0] + [int(c) for c in rng.integers(1, 2**31 - 1, total - 1)]
+    inputs = [base if c == 0 else torch.bitwise_xor(base, c) for c in consts]
+    alt = torch.empty_like(base)
+    params = gmms.make_params(K=args.K, mergers=args.mergers, prefetch=args.prefetch)
+    ws = gmms.Workspace()
+    stream = torch.cuda.current_stream()
+    K = args.K or 64
+    rounds = gmms.num_rounds(n, K)
+    # keep an untouched copy of input 0 for the bit-exact check
+    check_src = base.clone()
+
+    def step(i):
+        return gmms.sort_inplace(inputs[i], alt, params=params, workspace=ws, stream=stream)
+
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    gmms.stage_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.warmup, total):
+        step(i)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    stages = gmms.stage_times()
+    gmms.stage_timing(False)
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    gkeys = n / (ms_per_step * 1e-3) / 1e9
+
+    # per-stage averages (kind -> list of ms), merges grouped by round position
+    per_kind = {}
+    for kind, ms, _ in stages:
+        per_kind.setdefault(gmms.STAGE_NAMES[kind], []).append(ms)
+    merges = per_kind.get("merge", [])
+    merge_avg = sum(merges) / len(merges) if merges else float("nan")
+    merge_bytes = 2.0 * n * 4
+    achieved = merge_bytes / (merge_avg * 1e-3) / 1e9 if merges else None
+    per_round = []
+    if merges and rounds:
+        for r in range(rounds):
+            vals = merges[r::rounds]
+            per_round.append(round(sum(vals) / len(vals), 4))
+
+    # measured stream-copy peak (4 GiB, best of 10)
+    cp_n = (4 << 30) // 4
+    a = torch.empty(cp_n, dtype=torch.int32, device=dev)
+    b = torch.empty_like(a)
+    best = 1e9
+    for _ in range(10):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        gmms.copy(b, a, stream=stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    copy_peak = 2.0 * cp_n * 4 / (best * 1e-3) / 1e9
+    del a, b
+
+    # bit-exact check of input 0 against the CPU sort
+    k2 = check_src.clone()
+    k2a = torch.empty_like(k2)
+    res = gmms.sort_inplace(k2, k2a, params=params, workspace=ws, stream=stream)
+    got = (k2a if res else k2).cpu().numpy()
+    threads = oracle.lib.oracle_hw_threads()
+    want = host.copy()
+    oracle.lib.oracle_psort_i32(want.ctypes.data, n, threads)
+    bit_exact = bool(np.array_equal(got, want))
+
+    formula_bytes = 2.0 * n * 4 * (1 + rounds)
+    sort_gbps = formula_bytes / (ms_per_step * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        sample = min(n, 1 << 25)
+        t_single = oracle.lib.oracle_time_sort_i32(host.ctypes.data, sample, 1, 1)
+        t_par = oracle.lib.oracle_time_sort_i32(host.ctypes.data, n, threads, 1)
+        cpu = {
+            "value": round(n / t_par / 1e9, 4),
+            "unit": "GKeys/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"libstdc++ parallel sort of the same {n} keys, median of 1 after 1 warm-up",
+            "single_thread": {"value": round(sample / t_single / 1e9, 4), "unit": "GKeys/s", "cores": 1,
+                              "sample": f"std::sort of the first {sample} keys"},
+        }
+
+    out = {
+        "metric": METRIC,
+        "value": round(gkeys, 4),
+        "unit": "GKeys/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": f"synthetic: mt19937_64 seed {seed:#x}, key=(int32)(u64>>32); step i XORs a constant",
+        "config": {"workload": "plain sort, N=2^28 int32 uniform over the full int32 range",
+                   "n": n, "K": K, "rounds": rounds, "mergers": args.mergers or "auto"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2) if achieved else None,
+                     "peak": PUBLISHED_HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / PUBLISHED_HBM_PEAK_GBPS, 4) if achieved else None,
+                     "traffic": traffic_from_profiles(),
+                     "kernel": "k_merge (one K-way merge round)", "algorithmic_bytes_per_launch": merge_bytes,
+                     "avg_launch_ms": round(merge_avg, 4)},
+        "cpu_baseline": cpu,
+        "bit_exact_vs_std_sort": bit_exact,
+        "stream_copy_peak_GBps": round(copy_peak, 1),
+        "sort_roofline": {"bytes": formula_bytes, "achieved_GBps": round(sort_gbps, 1),
+                          "frac_of_copy_peak": round(sort_gbps / copy_peak, 4),
+                          "frac_of_published_peak": round(sort_gbps / PUBLISHED_HBM_PEAK_GBPS, 4)},
+        "stage_ms_avg": {k: round(sum(v) / len(v), 4) for k, v in per_kind.items()},
+        "merge_round_ms": per_round,
+    }
+    print(json.dumps(out), flush=True)
+    if not bit_exact:
+        sys.exit(1)
+
+
+if __name__ == "__main__":
+    main()

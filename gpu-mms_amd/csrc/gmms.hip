@@ -18,6 +18,7 @@
 #include "../../include/gpu_mms/gmms.h"
 #include "gmms_basecase.hxx"
 #include "gmms_merge.hxx"
+#include "gmms_tmerge.hxx"
 
 using namespace gmms;
 
@@ -26,7 +27,10 @@ namespace {
 struct Resolved {
   int K, PL, D, mergers;
   int64_t slack;
+  bool heap;  // pipelined minBlockHeap (k_merge) or tile merge (k_tmerge)
 };
+
+constexpr int TM_F = 64;  // tile-merge window per list
 
 int ilog2(int x) {
   int r = 0;
@@ -68,13 +72,18 @@ int resolve(int dtype, const gmms_params* p, Resolved* r, bool need_device) {
   int D = q.prefetch ? q.prefetch : d.prefetch;
   if (D < 1 || D > 8) return GMMS_ERROR_INVALID_VALUE;
   const int W = elem_words(dtype);
-  int K = q.K;
+  const bool heap = q.pipeline != 0;
+  auto lds_bytes = [&](int k) {
+    return (heap ? merge_lds_words(k, W, D) : tmerge_lds_words(k, W, TM_F)) * sizeof(uint32_t);
+  };
+  int K = q.kway;
   if (K == 0) {
     K = MAX_K;
-    while (K > 2 && merge_lds_words(K, W, D) * sizeof(uint32_t) > LDS_LIMIT) K >>= 1;
+    while (K > 2 && lds_bytes(K) > LDS_LIMIT) K >>= 1;
   }
   if (K < 2 || K > MAX_K || (K & (K - 1))) return GMMS_ERROR_INVALID_VALUE;
-  if (merge_lds_words(K, W, D) * sizeof(uint32_t) > LDS_LIMIT) return GMMS_ERROR_UNSUPPORTED;
+  if (lds_bytes(K) > LDS_LIMIT) return GMMS_ERROR_UNSUPPORTED;
+  r->heap = heap;
   if (q.mergers < 0 || q.slack < 0) return GMMS_ERROR_INVALID_VALUE;
   r->K = K;
   r->PL = ilog2(K);
@@ -118,14 +127,66 @@ size_t ws_bytes_for(int64_t n, const Resolved& r) {
   return need;
 }
 
-#define HIP_OK(x)                                  \
-  do {                                             \
-    if ((x) != hipSuccess) return GMMS_ERROR_HIP;  \
+thread_local char g_last_error[256] = "";
+
+int hip_fail(hipError_t e, const char* what) {
+  std::snprintf(g_last_error, sizeof(g_last_error), "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
+  return GMMS_ERROR_HIP;
+}
+
+#define HIP_OK(x)                                    \
+  do {                                               \
+    hipError_t e_ = (x);                             \
+    if (e_ != hipSuccess) return hip_fail(e_, #x);   \
   } while (0)
+
+// Opt-in per-launch timing: HIP events recorded on the launch stream around
+// every kernel, read back by gmms_stage_times().
+struct StageRec {
+  hipEvent_t a, b;
+  int kind;
+  int64_t elems;
+};
+struct StageTimer {
+  std::mutex mu;
+  bool on = false;
+  std::vector<StageRec> recs;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  hipEvent_t get() {
+    if (used == pool.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      pool.push_back(e);
+    }
+    return pool[used++];
+  }
+} g_timer;
+
+struct StageScope {
+  hipEvent_t b = nullptr;
+  hipStream_t s;
+  StageScope(int kind, int64_t elems, hipStream_t st) : s(st) {
+    if (!g_timer.on) return;
+    std::lock_guard<std::mutex> g(g_timer.mu);
+    hipEvent_t a = g_timer.get();
+    b = g_timer.get();
+    if (!a || !b) {
+      b = nullptr;
+      return;
+    }
+    (void)hipEventRecord(a, s);
+    g_timer.recs.push_back({a, b, kind, elems});
+  }
+  ~StageScope() {
+    if (b) (void)hipEventRecord(b, s);
+  }
+};
 
 template <int DT>
 int launch_basecase(void* keys, uint32_t* vals, int64_t n, bool cin, bool cout, hipStream_t s) {
   if (n <= 0) return GMMS_SUCCESS;
+  StageScope scope(GMMS_STAGE_BASECASE, n, s);
   const int64_t tiles = (n + TILE - 1) / TILE;
   const dim3 grid((unsigned)((tiles + BASE_WAVES - 1) / BASE_WAVES));
   const dim3 block(BASE_WAVES * WAVE);
@@ -142,6 +203,7 @@ int launch_partition(RoundArgs& a, hipStream_t s) {
   const int64_t waves = a.tasks * (a.m - 1);
   if (waves <= 0) return GMMS_SUCCESS;
   const dim3 grid((unsigned)((waves + PART_WAVES - 1) / PART_WAVES));
+  StageScope scope(GMMS_STAGE_PARTITION, a.n, s);
   hipLaunchKernelGGL((k_partition<DT>), grid, dim3(PART_WAVES * WAVE), 0, s, a);
   HIP_OK(hipGetLastError());
   return GMMS_SUCCESS;
@@ -154,6 +216,7 @@ int launch_merge(RoundArgs& a, bool cout, hipStream_t s) {
   if (lds > LDS_LIMIT) return GMMS_ERROR_UNSUPPORTED;
   const dim3 grid((unsigned)(a.tasks * a.m));
   const dim3 block(a.PL * WAVE);
+  StageScope scope(GMMS_STAGE_MERGE, a.n, s);
   if (cout) {
     static bool attr = false;
     if (!attr) {
@@ -168,6 +231,34 @@ int launch_merge(RoundArgs& a, bool cout, hipStream_t s) {
       attr = true;
     }
     hipLaunchKernelGGL((k_merge<DT, false>), grid, block, lds, s, a);
+  }
+  HIP_OK(hipGetLastError());
+  return GMMS_SUCCESS;
+}
+
+template <int DT>
+int launch_tmerge(RoundArgs& a, bool cout, hipStream_t s) {
+  using E = typename DType<DT>::E;
+  const size_t lds = tmerge_lds_words(a.K, E::W, TM_F) * sizeof(uint32_t);
+  if (lds > LDS_LIMIT) return GMMS_ERROR_UNSUPPORTED;
+  const dim3 grid((unsigned)(a.tasks * a.m));
+  StageScope scope(GMMS_STAGE_MERGE, a.n, s);
+  if (cout) {
+    static bool attr = false;
+    if (!attr) {
+      (void)hipFuncSetAttribute((const void*)k_tmerge<DT, true, TM_F>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)LDS_LIMIT);
+      attr = true;
+    }
+    hipLaunchKernelGGL((k_tmerge<DT, true, TM_F>), grid, dim3(TM_THREADS), lds, s, a);
+  } else {
+    static bool attr = false;
+    if (!attr) {
+      (void)hipFuncSetAttribute((const void*)k_tmerge<DT, false, TM_F>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)LDS_LIMIT);
+      attr = true;
+    }
+    hipLaunchKernelGGL((k_tmerge<DT, false, TM_F>), grid, dim3(TM_THREADS), lds, s, a);
   }
   HIP_OK(hipGetLastError());
   return GMMS_SUCCESS;
@@ -206,7 +297,7 @@ int run_round(const void* ik, const void* iv, void* ok, void* ov, int64_t n, int
     int st = launch_partition<DT>(a, s);
     if (st) return st;
   }
-  return launch_merge<DT>(a, cout, s);
+  return r.heap ? launch_merge<DT>(a, cout, s) : launch_tmerge<DT>(a, cout, s);
 }
 
 template <int DT>
@@ -242,6 +333,13 @@ int dispatch(int dtype, F&& f) {
   }
 }
 
+// Stream copy with 128-bit loads/stores: the measured HBM peak the roofline
+// is quoted against, and the calibration kernel for FETCH_SIZE/WRITE_SIZE.
+__global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 template <int DT>
 __global__ void k_pad(void* keys, uint32_t* vals, int64_t count) {
   using E = typename DType<DT>::E;
@@ -254,6 +352,44 @@ __global__ void k_pad(void* keys, uint32_t* vals, int64_t count) {
 extern "C" {
 
 const char* gmms_version(void) { return "gpu-mms-amd 0.1.0 gfx950"; }
+
+const char* gmms_last_error(void) { return g_last_error; }
+
+int gmms_copy(void* dst, const void* src, size_t bytes, void* stream) {
+  if (!dst || !src || (bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15)) return GMMS_ERROR_INVALID_VALUE;
+  if (!bytes) return GMMS_SUCCESS;
+  const int64_t n16 = (int64_t)(bytes / 16);
+  const int64_t blocks = std::min<int64_t>((n16 + 255) / 256, (int64_t)cu_count() * 32);
+  hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, (const uint4*)src,
+                     (uint4*)dst, n16);
+  HIP_OK(hipGetLastError());
+  return GMMS_SUCCESS;
+}
+
+int gmms_stage_timing(int enable) {
+  std::lock_guard<std::mutex> g(g_timer.mu);
+  g_timer.on = enable != 0;
+  g_timer.recs.clear();
+  g_timer.used = 0;
+  return GMMS_SUCCESS;
+}
+
+int gmms_stage_times(int* kinds, float* ms, int64_t* elems, int cap, int* count) {
+  if (!count || cap < 0) return GMMS_ERROR_INVALID_VALUE;
+  std::lock_guard<std::mutex> g(g_timer.mu);
+  const int n = (int)g_timer.recs.size();
+  *count = n;
+  for (int i = 0; i < n && i < cap; ++i) {
+    const StageRec& r = g_timer.recs[i];
+    HIP_OK(hipEventSynchronize(r.b));
+    float t = 0.f;
+    HIP_OK(hipEventElapsedTime(&t, r.a, r.b));
+    if (kinds) kinds[i] = r.kind;
+    if (ms) ms[i] = t;
+    if (elems) elems[i] = r.elems;
+  }
+  return GMMS_SUCCESS;
+}
 
 const char* gmms_status_string(int s) {
   switch (s) {
@@ -271,7 +407,7 @@ const char* gmms_status_string(int s) {
 int gmms_get_default_params(gmms_params* p) {
   if (!p) return GMMS_ERROR_INVALID_VALUE;
   std::memset(p, 0, sizeof(*p));
-  p->K = 0;  // auto: 64 for 4- and 8-byte keys, 32 for pairs
+  p->kway = 0;  // auto: 64 for 4- and 8-byte keys, 32 for pairs
   p->pipeline = 1;
   p->mergers = 0;
   p->prefetch = 2;
@@ -336,28 +472,43 @@ int gmms_sort(int dtype, void* keys, void* keys_alt, void* vals, void* vals_alt,
   });
 }
 
-int gmms_multimergesort(int dtype, void* input, void* output, void* host_array, int thread_blocks, int64_t n,
-                        void** result) {
+int gmms_multimergesort_params(int dtype, void* input, void* output, void* host_array, const gmms_params* pp,
+                               int64_t n, void** result) {
   (void)host_array;
   if (!result) return GMMS_ERROR_INVALID_VALUE;
   *result = input;
   if (!valid_dtype(dtype) || n < 0) return GMMS_ERROR_INVALID_VALUE;
   if (n == 0) return GMMS_SUCCESS;
+  if (!input || !output) return GMMS_ERROR_INVALID_VALUE;
   if (has_vals(dtype)) return GMMS_ERROR_UNSUPPORTED;  // pairs need the value buffers: use gmms_sort
   gmms_params p;
-  gmms_get_default_params(&p);
-  if (thread_blocks > 0) p.mergers = thread_blocks;
+  if (pp) p = *pp; else gmms_get_default_params(&p);
   size_t wsb = 0;
   int st = gmms_workspace_size(dtype, n, &p, &wsb);
   if (st) return st;
   void* ws = nullptr;
-  if (wsb && hipMalloc(&ws, wsb) != hipSuccess) return GMMS_ERROR_OUT_OF_MEMORY;
+  if (wsb) {
+    const hipError_t e = hipMalloc(&ws, wsb);
+    if (e != hipSuccess) {
+      hip_fail(e, "hipMalloc(workspace)");
+      return GMMS_ERROR_OUT_OF_MEMORY;
+    }
+  }
   int alt = 0;
   st = gmms_sort(dtype, input, output, nullptr, nullptr, n, &p, ws, wsb, nullptr, &alt);
-  if (hipDeviceSynchronize() != hipSuccess && !st) st = GMMS_ERROR_HIP;
+  const hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess && !st) st = hip_fail(e, "hipDeviceSynchronize");
   if (ws) (void)hipFree(ws);
   if (!st) *result = alt ? output : input;
   return st;
+}
+
+int gmms_multimergesort(int dtype, void* input, void* output, void* host_array, int thread_blocks, int64_t n,
+                        void** result) {
+  gmms_params p;
+  gmms_get_default_params(&p);
+  if (thread_blocks > 0) p.mergers = thread_blocks;
+  return gmms_multimergesort_params(dtype, input, output, host_array, &p, n, result);
 }
 
 int gmms_squaresort(int dtype, void* keys, void* vals, int64_t n, int to_internal, int to_external, void* stream) {
@@ -416,7 +567,8 @@ int gmms_pad(int dtype, void* keys, void* vals, int64_t count, void* stream) {
     constexpr int DT = decltype(dt)::value;
     hipLaunchKernelGGL((k_pad<DT>), dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, keys, (uint32_t*)vals,
                        count);
-    return hipGetLastError() == hipSuccess ? (int)GMMS_SUCCESS : (int)GMMS_ERROR_HIP;
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? (int)GMMS_SUCCESS : hip_fail(e, "k_pad launch");
   });
 }
 

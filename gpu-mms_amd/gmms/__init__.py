@@ -50,14 +50,17 @@ TILE = 1024  # base-case tile (the reference's M, src/params.h:36)
 class GmmsError(RuntimeError):
     def __init__(self, status: int, where: str):
         self.status = status
-        super().__init__(f"{where}: {status_string(status)} (status {status})")
+        detail = ""
+        if status == ERROR_HIP:
+            detail = " -- " + lib().gmms_last_error().decode()
+        super().__init__(f"{where}: {status_string(status)} (status {status}){detail}")
 
 
 class Params(ctypes.Structure):
     """gmms_params (gmms.h); replaces src/params.h:18-38 (K, PIPELINE, BLOCKS)."""
 
     _fields_ = [
-        ("K", ctypes.c_int32),
+        ("kway", ctypes.c_int32),
         ("pipeline", ctypes.c_int32),
         ("mergers", ctypes.c_int32),
         ("prefetch", ctypes.c_int32),
@@ -76,12 +79,22 @@ def lib() -> ctypes.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"libgmms.so not built at {LIB_PATH}; run __graft_entry__.build()")
+    # PyTorch bundles its own HIP runtime under the same soname
+    # (libamdhip64.so.7). Load it first so libgmms.so binds to that one
+    # instead of pulling /opt/rocm's into the same process: two HIP runtimes
+    # in one process do not share devices, streams or allocations.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, i64, i32, sz = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_size_t
     P = ctypes.POINTER(Params)
     sig = {
         "gmms_version": (ctypes.c_char_p, []),
         "gmms_status_string": (ctypes.c_char_p, [i32]),
+        "gmms_last_error": (ctypes.c_char_p, []),
+        "gmms_multimergesort_params": (i32, [i32, vp, vp, vp, P, i64, ctypes.POINTER(vp)]),
         "gmms_get_default_params": (i32, [P]),
         "gmms_element_size": (i32, [i32, ctypes.POINTER(sz), ctypes.POINTER(sz)]),
         "gmms_num_rounds": (i32, [i64, i32, ctypes.POINTER(i32)]),
@@ -93,6 +106,9 @@ def lib() -> ctypes.CDLL:
         "gmms_merge_round": (i32, [i32, vp, vp, vp, vp, i64, i64, P, i32, vp, sz, vp]),
         "gmms_find_partitions": (i32, [i32, vp, vp, i64, i64, P, vp, vp, i64, ctypes.POINTER(i32), vp]),
         "gmms_pad": (i32, [i32, vp, vp, i64, vp]),
+        "gmms_stage_timing": (i32, [i32]),
+        "gmms_copy": (i32, [vp, vp, sz, vp]),
+        "gmms_stage_times": (i32, [vp, vp, vp, i32, ctypes.POINTER(i32)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -128,10 +144,14 @@ def default_params() -> Params:
     return p
 
 
-def make_params(K: int = 0, mergers: int = 0, prefetch: int = 0, slack: int = 0) -> Params:
+def make_params(K: int = 0, mergers: int = 0, prefetch: int = 0, slack: int = 0,
+                pipeline: Optional[int] = None) -> Params:
+    """pipeline=1: pipelined minBlockHeap merge; pipeline=0: tile merge."""
     p = default_params()
+    if pipeline is not None:
+        p.pipeline = int(pipeline)
     if K:
-        p.K = K
+        p.kway = K
     if mergers:
         p.mergers = mergers
     if prefetch:
@@ -294,7 +314,7 @@ def find_partitions(keys, list_len: int, values=None, *, dtype: Optional[int] = 
     _check_dev(keys, values)
     dt = dtype if dtype is not None else (PAIR_U64_U32 if values is not None else dtype_of(keys))
     p = params if params is not None else default_params()
-    K = p.K or 64
+    K = p.kway or 64
     cuts = torch.zeros(cap_mergers * K, dtype=torch.int64, device=keys.device)
     skips = torch.zeros(cap_mergers, dtype=torch.int64, device=keys.device)
     m = ctypes.c_int(0)
@@ -309,3 +329,35 @@ def pad(keys, count: int, values=None, *, dtype: Optional[int] = None, stream=No
     _check_dev(keys, values)
     dt = dtype if dtype is not None else (PAIR_U64_U32 if values is not None else dtype_of(keys))
     _check(lib().gmms_pad(dt, _ptr(keys), _ptr(values), count, _stream_ptr(stream)), "gmms_pad")
+
+
+def copy(dst, src, stream=None) -> None:
+    """128-bit stream copy of src into dst (the HBM peak probe)."""
+    _check_dev(dst, src)
+    nbytes = src.numel() * src.element_size()
+    _check(lib().gmms_copy(_ptr(dst), _ptr(src), nbytes, _stream_ptr(stream)), "gmms_copy")
+
+
+# ------------------------------------------------------------ stage timing
+STAGE_BASECASE, STAGE_PARTITION, STAGE_MERGE = 0, 1, 2
+STAGE_NAMES = {STAGE_BASECASE: "basecase", STAGE_PARTITION: "partition", STAGE_MERGE: "merge"}
+
+
+def stage_timing(enable: bool) -> None:
+    """Record HIP events around every library kernel launch (on its stream)."""
+    _check(lib().gmms_stage_timing(int(enable)), "gmms_stage_timing")
+
+
+def stage_times():
+    """[(stage_kind, ms, elements)] for the launches recorded since stage_timing()."""
+    import numpy as np
+
+    cnt = ctypes.c_int(0)
+    _check(lib().gmms_stage_times(None, None, None, 0, ctypes.byref(cnt)), "gmms_stage_times")
+    n = cnt.value
+    kinds = np.zeros(max(n, 1), np.int32)
+    ms = np.zeros(max(n, 1), np.float32)
+    el = np.zeros(max(n, 1), np.int64)
+    _check(lib().gmms_stage_times(kinds.ctypes.data, ms.ctypes.data, el.ctypes.data, n, ctypes.byref(cnt)),
+           "gmms_stage_times")
+    return [(int(kinds[i]), float(ms[i]), int(el[i])) for i in range(n)]

@@ -55,7 +55,7 @@ typedef enum gmms_dtype {
  * BLOCKS (= threadBlocks, the merger count when a round has few tasks).
  * Zero fields take the defaults of gmms_get_default_params. */
 typedef struct gmms_params {
-  int32_t K;         /* lists merged per round; 0: auto = the largest power
+  int32_t kway;      /* K: lists merged per round; 0: auto = the largest power
                         of two <= 64 whose heap fits LDS (64 for 4- and
                         8-byte keys, 32 for pairs)                           */
   int32_t pipeline;  /* 1: one wave per heap level (the only kernel)          */
@@ -69,6 +69,9 @@ typedef struct gmms_params {
 const char* gmms_version(void);
 /* Human-readable name of a status code. */
 const char* gmms_status_string(int status);
+/* The HIP call and error behind the calling thread's last GMMS_ERROR_HIP
+ * ("" if none). */
+const char* gmms_last_error(void);
 /* Fill *p with the defaults (K auto, PIPELINE on, auto mergers). */
 int gmms_get_default_params(gmms_params* p);
 /* Bytes per key and per value (0 for key-only dtypes) of a dtype. */
@@ -107,6 +110,26 @@ int gmms_sort(int dtype, void* keys, void* keys_alt, void* vals, void* vals_alt,
  * the reference (it is a debugging aid there). thread_blocks <= 0: auto. */
 int gmms_multimergesort(int dtype, void* input, void* output, void* host_array,
                         int thread_blocks, int64_t n, void** result);
+/* The same with every knob of src/params.h:18-38 (K, PIPELINE, BLOCKS as
+ * p->mergers) instead of threadBlocks alone; used by multimergesort.hxx. */
+int gmms_multimergesort_params(int dtype, void* input, void* output, void* host_array,
+                               const gmms_params* p, int64_t n, void** result);
+
+/* Device-to-device copy with 128-bit accesses (bytes and both pointers
+ * 16-byte aligned). No counterpart in the reference: it is the stream-copy
+ * probe that the HBM roofline of the sort is measured against. */
+int gmms_copy(void* dst, const void* src, size_t bytes, void* stream);
+
+/* ---- per-launch timing (the reference times the whole call with CUDA
+ * events, src/testMM.cu:171-186) ---------------------------------------- */
+enum { GMMS_STAGE_BASECASE = 0, GMMS_STAGE_PARTITION = 1, GMMS_STAGE_MERGE = 2 };
+/* Start (enable=1) or stop (0) recording HIP events around every kernel the
+ * library launches, on the launch's own stream; either call clears the list. */
+int gmms_stage_timing(int enable);
+/* Read the recorded launches (waits for their events): kinds[i] is a
+ * GMMS_STAGE_*, ms[i] its duration, elems[i] the elements it covered.
+ * *count receives the number recorded (entries beyond cap are skipped). */
+int gmms_stage_times(int* kinds, float* ms, int64_t* elems, int cap, int* count);
 
 /* ---- single stages, for tests and for callers that drive rounds ------- */
 

@@ -1,0 +1,78 @@
+/*
+ * multimergesort.hxx -- the drop-in entry point of the reference,
+ * multimergesort<T,cmp>(input, output, host_array, threadBlocks, N)
+ * (README.md:16-25) and its 4-argument code form
+ * (src/multimergesort.hxx:56-57), as host C++ over the C ABI of libgmms.so.
+ *
+ * Contract kept from the reference: `input` and `output` are device arrays of
+ * at least N elements, both are clobbered, and the returned pointer is the
+ * one holding the sorted data (the base case sorts `input` in place and the
+ * merge rounds ping-pong, src/multimergesort.hxx:128,140). The call is
+ * synchronous and allocates its partition workspace itself, like the
+ * reference (:61-63,137-138). host_array is unused, as on the reference's
+ * live path. On error it returns nullptr (the reference cannot fail
+ * visibly); gmms_last_error() names the HIP failure.
+ *
+ * Callers that need a stream, a caller-owned workspace, pairs or 64-bit N use
+ * gmms_sort() from gmms.h directly.
+ *
+ * Link with -lgmms (gpu-mms_amd/gmms/libgmms.so); no HIP compiler is needed
+ * to use this header.
+ */
+#ifndef GPU_MMS_MULTIMERGESORT_HXX
+#define GPU_MMS_MULTIMERGESORT_HXX
+
+#include <cstdint>
+#include <type_traits>
+
+#include "cmp.hxx"
+#include "gmms.h"
+#include "params.h"
+
+namespace gpu_mms {
+
+template <typename T>
+struct dtype_of;
+template <> struct dtype_of<int32_t> { static constexpr int value = GMMS_INT32; };
+template <> struct dtype_of<uint32_t> { static constexpr int value = GMMS_UINT32; };
+template <> struct dtype_of<float> { static constexpr int value = GMMS_FLOAT32; };
+template <> struct dtype_of<int64_t> { static constexpr int value = GMMS_INT64; };
+template <> struct dtype_of<uint64_t> { static constexpr int value = GMMS_UINT64; };
+template <> struct dtype_of<double> { static constexpr int value = GMMS_FLOAT64; };
+
+inline int last_status_storage(int set = -1) {
+  static thread_local int s = GMMS_SUCCESS;
+  if (set >= 0) s = set;
+  return s;
+}
+
+}  // namespace gpu_mms
+
+/* Status of the calling thread's last multimergesort call. */
+inline int multimergesort_status() { return gpu_mms::last_status_storage(); }
+
+template <typename T, fptr_t f>
+T* multimergesort(T* input, T* output, T* h_data, int threadBlocks, int64_t N) {
+  static_assert(std::is_same<T, DATATYPE>::value, "T must be DATATYPE (cmp.hxx)");
+  static_assert(f == &cmp<DATATYPE>, "only cmp<DATATYPE> (a < b) runs on libgmms.so");
+  void* result = nullptr;
+  gmms_params p;
+  gmms_get_default_params(&p);
+  p.kway = K;
+#ifdef PIPELINE
+  p.pipeline = 1;
+#else
+  p.pipeline = 0;
+#endif
+  p.mergers = threadBlocks > 0 ? threadBlocks : BLOCKS;
+  const int st = gmms_multimergesort_params(gpu_mms::dtype_of<T>::value, input, output, h_data, &p, N, &result);
+  gpu_mms::last_status_storage(st);
+  return st == GMMS_SUCCESS ? static_cast<T*>(result) : nullptr;
+}
+
+template <typename T, fptr_t f>
+T* multimergesort(T* input, T* output, T* h_data, int N) {
+  return multimergesort<T, f>(input, output, h_data, BLOCKS, (int64_t)N);
+}
+
+#endif

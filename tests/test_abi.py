@@ -31,7 +31,7 @@ def test_version_and_status_strings(gmms):
 
 def test_default_params(gmms):
     p = gmms.default_params()
-    assert (p.K, p.pipeline, p.mergers) == (0, 1, 0)  # K=0: auto (64, or 32 for pairs)
+    assert (p.kway, p.pipeline, p.mergers) == (0, 1, 0)  # K=0: auto (64, or 32 for pairs)
     assert 1 <= p.prefetch <= 8
 
 
@@ -71,9 +71,9 @@ def test_argument_validation_without_gpu(gmms):
     assert L.gmms_sort(gmms.PAIR_U64_U32, fake, fake, None, None, 10, None, None, 0, None, None) == 1
     assert L.gmms_sort(gmms.INT32, fake, fake, fake, fake, 10, None, None, 0, None, None) == 1
     bad = gmms.make_params()
-    bad.K = 48
+    bad.kway = 48
     assert L.gmms_sort(0, fake, fake, None, None, 10, ctypes.byref(bad), None, 0, None, None) == 1
-    bad.K = 128
+    bad.kway = 128
     assert L.gmms_sort(0, fake, fake, None, None, 10, ctypes.byref(bad), None, 0, None, None) == 1
     # a sort that needs partitions but gets no workspace is refused up front
     p = gmms.make_params(K=64, mergers=512)
