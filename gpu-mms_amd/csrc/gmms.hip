@@ -19,6 +19,7 @@
 #include "gmms_basecase.hxx"
 #include "gmms_merge.hxx"
 #include "gmms_tmerge.hxx"
+#include "gmms_wmerge.hxx"
 
 using namespace gmms;
 
@@ -27,8 +28,13 @@ namespace {
 struct Resolved {
   int K, PL, D, mergers;
   int64_t slack;
-  bool heap;  // pipelined minBlockHeap (k_merge) or tile merge (k_tmerge)
+  int kernel;  // GMMS_MERGE_* : wavefront tile, pipelined heap, workgroup tile
 };
+
+enum { GMMS_MERGE_WAVE = 0, GMMS_MERGE_HEAP = 1, GMMS_MERGE_TILE = 2 };
+
+// wavefront-merger window budget (elements): 13 KB of LDS per wavefront
+constexpr int wm_win(int W) { return W == 1 ? 1024 : 512; }
 
 constexpr int TM_F = 64;  // tile-merge window per list
 
@@ -72,9 +78,14 @@ int resolve(int dtype, const gmms_params* p, Resolved* r, bool need_device) {
   int D = q.prefetch ? q.prefetch : d.prefetch;
   if (D < 1 || D > 8) return GMMS_ERROR_INVALID_VALUE;
   const int W = elem_words(dtype);
-  const bool heap = q.pipeline != 0;
-  auto lds_bytes = [&](int k) {
-    return (heap ? merge_lds_words(k, W, D) : tmerge_lds_words(k, W, TM_F)) * sizeof(uint32_t);
+  if (q.pipeline < 0 || q.pipeline > 2) return GMMS_ERROR_INVALID_VALUE;
+  const int kern = q.pipeline;
+  auto lds_bytes = [&](int k) -> size_t {
+    if (kern == GMMS_MERGE_HEAP) return merge_lds_words(k, W, D) * sizeof(uint32_t);
+    if (kern == GMMS_MERGE_TILE) return tmerge_lds_words(k, W, TM_F) * sizeof(uint32_t);
+    // wavefront merger: window per list WIN/K must stay >= 16 elements
+    return wm_win(W) / k >= 16 ? WM_WAVES * wmerge_lds_words_per_wave(W, wm_win(W)) * sizeof(uint32_t)
+                               : LDS_LIMIT + 1;
   };
   int K = q.kway;
   if (K == 0) {
@@ -83,12 +94,14 @@ int resolve(int dtype, const gmms_params* p, Resolved* r, bool need_device) {
   }
   if (K < 2 || K > MAX_K || (K & (K - 1))) return GMMS_ERROR_INVALID_VALUE;
   if (lds_bytes(K) > LDS_LIMIT) return GMMS_ERROR_UNSUPPORTED;
-  r->heap = heap;
+  r->kernel = kern;
   if (q.mergers < 0 || q.slack < 0) return GMMS_ERROR_INVALID_VALUE;
   r->K = K;
   r->PL = ilog2(K);
   r->D = D;
-  r->mergers = q.mergers ? q.mergers : (need_device ? 2 * cu_count() : 512);
+  // resident mergers per CU: heap 2 (LDS), workgroup tile 3, wavefront tile 12
+  const int per_cu = kern == GMMS_MERGE_HEAP ? 2 : (kern == GMMS_MERGE_TILE ? 3 : 12);
+  r->mergers = q.mergers ? q.mergers : per_cu * (need_device ? cu_count() : 256);
   r->slack = q.slack;
   return GMMS_SUCCESS;
 }
@@ -264,6 +277,32 @@ int launch_tmerge(RoundArgs& a, bool cout, hipStream_t s) {
   return GMMS_SUCCESS;
 }
 
+template <int DT, bool CONV, int WIN>
+int launch_wmerge_win(RoundArgs& a, hipStream_t s) {
+  using E = typename DType<DT>::E;
+  const size_t lds = WM_WAVES * wmerge_lds_words_per_wave(E::W, WIN) * sizeof(uint32_t);
+  if (lds > LDS_LIMIT) return GMMS_ERROR_UNSUPPORTED;
+  const int64_t mergers = a.tasks * a.m;
+  const dim3 grid((unsigned)((mergers + WM_WAVES - 1) / WM_WAVES));
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)k_wmerge<DT, CONV, WIN>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)LDS_LIMIT);
+    attr = true;
+  }
+  hipLaunchKernelGGL((k_wmerge<DT, CONV, WIN>), grid, dim3(WM_WAVES * WAVE), lds, s, a);
+  HIP_OK(hipGetLastError());
+  return GMMS_SUCCESS;
+}
+
+template <int DT>
+int launch_wmerge(RoundArgs& a, bool cout, hipStream_t s) {
+  using E = typename DType<DT>::E;
+  constexpr int WIN = wm_win(E::W);
+  StageScope scope(GMMS_STAGE_MERGE, a.n, s);
+  return cout ? launch_wmerge_win<DT, true, WIN>(a, s) : launch_wmerge_win<DT, false, WIN>(a, s);
+}
+
 RoundArgs round_args(const void* ik, const void* iv, void* ok, void* ov, int64_t n, int64_t L,
                      const Resolved& r, int m, void* ws) {
   RoundArgs a{};
@@ -297,7 +336,9 @@ int run_round(const void* ik, const void* iv, void* ok, void* ov, int64_t n, int
     int st = launch_partition<DT>(a, s);
     if (st) return st;
   }
-  return r.heap ? launch_merge<DT>(a, cout, s) : launch_tmerge<DT>(a, cout, s);
+  if (r.kernel == GMMS_MERGE_HEAP) return launch_merge<DT>(a, cout, s);
+  if (r.kernel == GMMS_MERGE_TILE) return launch_tmerge<DT>(a, cout, s);
+  return launch_wmerge<DT>(a, cout, s);
 }
 
 template <int DT>

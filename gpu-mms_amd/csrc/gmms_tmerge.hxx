@@ -119,12 +119,21 @@ __global__ __launch_bounds__(TM_THREADS) void k_tmerge(RoundArgs a) {
       want[lane] = (int)(rem < F ? rem : F);
     }
     __syncthreads();
-    for (int k = wave; k < K; k += TM_WAVES) {
-      const int c = cnt[k], wk = want[k], hk = head[k];
-      const int64_t ck = cur[k];
-      for (int i = c + lane; i < wk; i += WAVE) {
-        const E v = gload<DT, false>(a.in_keys, a.in_vals, ck + i);
-        pl_put<E>(win, cap, k * F + ((hk + i) & (F - 1)), v);
+    {
+      // every load of this wave in flight before the first LDS store
+      constexpr int PER = MAX_K / TM_WAVES * (F / WAVE);
+      E v[PER];
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        const int k = wave + (u / (F / WAVE)) * TM_WAVES;
+        const int i = (u % (F / WAVE)) * WAVE + lane;
+        if (k < K && i >= cnt[k] && i < want[k]) v[u] = gload<DT, false>(a.in_keys, a.in_vals, cur[k] + i);
+      }
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        const int k = wave + (u / (F / WAVE)) * TM_WAVES;
+        const int i = (u % (F / WAVE)) * WAVE + lane;
+        if (k < K && i >= cnt[k] && i < want[k]) pl_put<E>(win, cap, k * F + ((head[k] + i) & (F - 1)), v[u]);
       }
     }
     __syncthreads();
@@ -276,15 +285,9 @@ __global__ __launch_bounds__(TM_THREADS) void k_tmerge(RoundArgs a) {
 
     // ---- 4. gather taken prefixes into bufA (runs at off[k])
     const int S = sh[0];
-    for (int o = tid; o < S; o += TM_THREADS) {
-      int l = 0, h = K - 1;  // largest k with off[k] <= o
-      while (l < h) {
-        const int mid = (l + h + 1) >> 1;
-        if (off[mid] <= o) l = mid; else h = mid - 1;
-      }
-      const int k = l;
-      const int i = o - off[k];
-      pl_put<E>(bufA, cap, o, pl_get<E>(win, cap, k * F + ((head[k] + i) & (F - 1))));
+    for (int k = wave; k < K; k += TM_WAVES) {
+      const int sk = taken[k], ok = off[k], hk = head[k];
+      for (int i = lane; i < sk; i += WAVE) pl_put<E>(bufA, cap, ok + i, pl_get<E>(win, cap, k * F + ((hk + i) & (F - 1))));
     }
     __syncthreads();
 
@@ -300,7 +303,9 @@ __global__ __launch_bounds__(TM_THREADS) void k_tmerge(RoundArgs a) {
         int l = 0, h = K / g - 1;
         while (l < h) {
           const int mid = (l + h + 1) >> 1;
-          if (off[mid * g] <= o) l = mid; else h = mid - 1;
+          const bool c = off[mid * g] <= o;
+          l = c ? mid : l;
+          h = c ? h : mid - 1;
         }
         const int gs = off[l * g], gm = off[l * g + g / 2], ge = off[l * g + g];
         const int n1 = gm - gs, n2 = ge - gm;
@@ -310,24 +315,26 @@ __global__ __launch_bounds__(TM_THREADS) void k_tmerge(RoundArgs a) {
         while (lo < hi) {
           const int mid = (lo + hi) >> 1;
           // A1[mid] is among the first d outputs iff A1[mid] <= A2[d-1-mid] (left run first on ties)
-          if (!lt(pl_get<E>(src, cap, gm + d - 1 - mid), pl_get<E>(src, cap, gs + mid))) lo = mid + 1;
-          else hi = mid;
+          const bool c = !lt(pl_get<E>(src, cap, gm + d - 1 - mid), pl_get<E>(src, cap, gs + mid));
+          lo = c ? mid + 1 : lo;
+          hi = c ? hi : mid;
         }
         int i = lo, jj = d - lo;
-        const int stop = min(oend, ge);
-        E va = i < n1 ? pl_get<E>(src, cap, gs + i) : Limits<E>::hi();
-        E vb = jj < n2 ? pl_get<E>(src, cap, gm + jj) : Limits<E>::hi();
-        for (; o < stop; ++o) {
-          const bool ta = jj >= n2 || (i < n1 && !lt(vb, va));
-          pl_put<E>(dst, cap, o, ta ? va : vb);
-          if (ta) {
-            ++i;
-            if (i < n1) va = pl_get<E>(src, cap, gs + i);
-          } else {
-            ++jj;
-            if (jj < n2) vb = pl_get<E>(src, cap, gm + jj);
-          }
+        const int cntm = min(oend, ge) - o;
+        // Reads past a run's end land on the next run or on the LDS words
+        // after the buffer (index <= cap): in bounds, and never selected.
+        E va = pl_get<E>(src, cap, gs + i);
+        E vb = pl_get<E>(src, cap, gm + jj);
+        for (int e = 0; e < cntm; ++e) {
+          const bool ta = (jj >= n2) | ((i < n1) & !lt(vb, va));
+          pl_put<E>(dst, cap, o + e, ta ? va : vb);
+          i += ta ? 1 : 0;
+          jj += ta ? 0 : 1;
+          const E nv = pl_get<E>(src, cap, ta ? gs + i : gm + jj);
+          va = ta ? nv : va;
+          vb = ta ? vb : nv;
         }
+        o += cntm;
       }
       __syncthreads();
       uint32_t* t = src;

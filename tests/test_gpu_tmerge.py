@@ -1,10 +1,16 @@
-"""GPU parity of the tile merge kernel (pipeline=0) against the oracle,
+"""GPU parity of the two tile merge kernels -- pipeline=0 (one merger per
+wavefront) and pipeline=2 (one merger per workgroup) -- against the oracle,
 bit-exact: the same cases as the heap kernel plus presorted and clustered
-inputs that drive its galloping run copies."""
+inputs that drive their galloping run copies."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(params=[0, 2], ids=["wave", "workgroup"])
+def kern(request):
+    return request.param
 
 
 def _to_dev(a, dev):
@@ -13,29 +19,29 @@ def _to_dev(a, dev):
     return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
 
 
-def _sort(gmms, a, dev, **kw):
-    return gmms.sort(_to_dev(a, dev), params=gmms.make_params(pipeline=0, **kw)).cpu().numpy()
+def _sort(gmms, a, dev, kern, **kw):
+    return gmms.sort(_to_dev(a, dev), params=gmms.make_params(pipeline=kern, **kw)).cpu().numpy()
 
 
 @pytest.mark.parametrize("n", [1025, 5000, 65537, 300_001, (1 << 20) + 9000])
-def test_tmerge_int32(gmms, oracle, dev, n):
-    print(f"tmerge int32 n={n}", flush=True)
+def test_tmerge_int32(gmms, oracle, dev, kern, n):
+    print(f"kern={kern} tmerge int32 n={n}", flush=True)
     a = oracle.gen_i32(0x6D6D7300 + n, n, 32)
-    assert np.array_equal(_sort(gmms, a, dev), oracle.sort(a))
+    assert np.array_equal(_sort(gmms, a, dev, kern), oracle.sort(a))
 
 
 @pytest.mark.parametrize("K", [2, 4, 8, 16, 32, 64])
-def test_tmerge_branching_factors(gmms, oracle, dev, K):
+def test_tmerge_branching_factors(gmms, oracle, dev, kern, K):
     n = 200_003
-    print(f"tmerge K={K}", flush=True)
+    print(f"kern={kern} tmerge K={K}", flush=True)
     a = oracle.gen_i32(77 + K, n, 34)
-    assert np.array_equal(_sort(gmms, a, dev, K=K), oracle.sort(a))
+    assert np.array_equal(_sort(gmms, a, dev, kern, K=K), oracle.sort(a))
 
 
 @pytest.mark.parametrize("kind", ["dup1024", "allsame", "sorted", "reverse", "extremes", "blocks"])
-def test_tmerge_patterns(gmms, oracle, dev, kind):
+def test_tmerge_patterns(gmms, oracle, dev, kern, kind):
     n = 700_001
-    print(f"tmerge pattern {kind}", flush=True)
+    print(f"kern={kern} tmerge pattern {kind}", flush=True)
     if kind == "dup1024":
         a = oracle.gen_i32(3, n, 54)
     elif kind == "allsame":
@@ -51,40 +57,40 @@ def test_tmerge_patterns(gmms, oracle, dev, kind):
     else:  # sorted runs of 3000 at random offsets: long single-list stretches
         a = np.concatenate([np.sort(oracle.gen_i32(i, 3000, 40)) + i * 7 for i in range(n // 3000 + 1)])[:n]
         a = a.astype(np.int32)
-    assert np.array_equal(_sort(gmms, a, dev), oracle.sort(a))
+    assert np.array_equal(_sort(gmms, a, dev, kern), oracle.sort(a))
 
 
 @pytest.mark.parametrize("mergers", [1, 7, 512, 3000])
-def test_tmerge_merger_counts(gmms, oracle, dev, mergers):
+def test_tmerge_merger_counts(gmms, oracle, dev, kern, mergers):
     n = 1 << 20
-    print(f"tmerge mergers={mergers}", flush=True)
+    print(f"kern={kern} tmerge mergers={mergers}", flush=True)
     a = oracle.gen_i32(5, n, 54)
-    assert np.array_equal(_sort(gmms, a, dev, K=16, mergers=mergers), oracle.sort(a))
+    assert np.array_equal(_sort(gmms, a, dev, kern, K=16, mergers=mergers), oracle.sort(a))
 
 
-def test_tmerge_float_and_64bit(gmms, oracle, dev):
+def test_tmerge_float_and_64bit(gmms, oracle, dev, kern):
     import torch
 
     print("tmerge float32/uint64", flush=True)
     x = oracle.gen_f32_normal(8, 262_147)
     x[:10] = [-0.0, 0.0, np.inf, -np.inf, np.nan, -np.nan, 1e-45, -1e-45, 3.0, -3.0]
-    got = _sort(gmms, x, dev)
+    got = _sort(gmms, x, dev, kern)
     assert np.array_equal(got.view(np.uint32), oracle.sort(x).view(np.uint32))
     u = oracle.gen_u64(21, 150_001)
     t = torch.from_numpy(u.view(np.int64).copy()).to(dev)
     alt = torch.empty_like(t)
-    res = gmms.sort_inplace(t, alt, dtype=gmms.UINT64, params=gmms.make_params(pipeline=0))
+    res = gmms.sort_inplace(t, alt, dtype=gmms.UINT64, params=gmms.make_params(pipeline=kern))
     assert np.array_equal((alt if res else t).cpu().numpy().view(np.uint64), oracle.sort(u))
 
 
 @pytest.mark.parametrize("dist", [0, 1, 2, 3])
-def test_tmerge_pairs(gmms, oracle, dev, dist):
+def test_tmerge_pairs(gmms, oracle, dev, kern, dist):
     import torch
 
     n = 300_000
-    print(f"tmerge pairs dist={dist}", flush=True)
+    print(f"kern={kern} tmerge pairs dist={dist}", flush=True)
     k, v = oracle.gen_pairs(2000 + dist, n, dist, 1.1, 1 << 20)
-    p = gmms.make_params(pipeline=0)
+    p = gmms.make_params(pipeline=kern)
     ks, vs = gmms.sort(torch.from_numpy(k.view(np.int64)).to(dev), torch.from_numpy(v.view(np.int32)).to(dev),
                        params=p)
     wk, wv = oracle.sort_pairs(k, v)
