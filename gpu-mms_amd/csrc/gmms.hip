@@ -89,7 +89,9 @@ int resolve(int dtype, const gmms_params* p, Resolved* r, bool need_device) {
   };
   int K = q.kway;
   if (K == 0) {
-    K = MAX_K;
+    // wavefront merger: 16 lists (64- or 32-element windows); the others:
+    // the largest power of two that fits LDS
+    K = kern == GMMS_MERGE_WAVE ? 16 : MAX_K;
     while (K > 2 && lds_bytes(K) > LDS_LIMIT) K >>= 1;
   }
   if (K < 2 || K > MAX_K || (K & (K - 1))) return GMMS_ERROR_INVALID_VALUE;
@@ -448,8 +450,8 @@ const char* gmms_status_string(int s) {
 int gmms_get_default_params(gmms_params* p) {
   if (!p) return GMMS_ERROR_INVALID_VALUE;
   std::memset(p, 0, sizeof(*p));
-  p->kway = 0;  // auto: 64 for 4- and 8-byte keys, 32 for pairs
-  p->pipeline = 1;
+  p->kway = 0;      // auto (per merge kernel and dtype)
+  p->pipeline = 0;  // wavefront tile merge
   p->mergers = 0;
   p->prefetch = 2;
   p->slack = 0;

@@ -121,19 +121,25 @@ __global__ __launch_bounds__(TM_THREADS) void k_tmerge(RoundArgs a) {
     __syncthreads();
     {
       // every load of this wave in flight before the first LDS store
+      // unconditional loads (element 0 when the slot needs nothing): no
+      // branch join between them, so they are all in flight together
       constexpr int PER = MAX_K / TM_WAVES * (F / WAVE);
       E v[PER];
+      uint32_t vmask = 0;
 #pragma unroll
       for (int u = 0; u < PER; ++u) {
         const int k = wave + (u / (F / WAVE)) * TM_WAVES;
         const int i = (u % (F / WAVE)) * WAVE + lane;
-        if (k < K && i >= cnt[k] && i < want[k]) v[u] = gload<DT, false>(a.in_keys, a.in_vals, cur[k] + i);
+        const int kk = k < K ? k : 0;
+        const bool ok = k < K && i >= cnt[kk] && i < want[kk];
+        v[u] = gload<DT, false>(a.in_keys, a.in_vals, ok ? cur[kk] + i : 0);
+        vmask |= ok ? (1u << u) : 0u;
       }
 #pragma unroll
       for (int u = 0; u < PER; ++u) {
         const int k = wave + (u / (F / WAVE)) * TM_WAVES;
         const int i = (u % (F / WAVE)) * WAVE + lane;
-        if (k < K && i >= cnt[k] && i < want[k]) pl_put<E>(win, cap, k * F + ((head[k] + i) & (F - 1)), v[u]);
+        if ((vmask >> u) & 1u) pl_put<E>(win, cap, k * F + ((head[k] + i) & (F - 1)), v[u]);
       }
     }
     __syncthreads();

@@ -58,6 +58,68 @@ __device__ __forceinline__ void lput(uint32_t* base, int cap, int idx, const E& 
   for (int q = 0; q < E::W; ++q) base[q * cap + idx] = word(e, q);
 }
 
+// One stable pairwise merge pass: groups of g runs (run r at s_off[r]) are
+// merged in halves; lane l writes outputs [l*Eo, (l+1)*Eo). FROM_WIN: the
+// runs are list prefixes in the circular windows (run r = window r from
+// s_head[r]) instead of being packed in src. WM_CAP: the word-plane stride.
+// Reads one past a run's end stay inside this merger's LDS (index <= WM_CAP)
+// and are never selected.
+template <class E, bool FROM_WIN, int WM_CAP>
+__device__ __forceinline__ void merge_pass(const uint32_t* src, uint32_t* dst, int g, int Kp, int S, int Eo, int lane,
+                                           const int* s_off, const int* s_head, int F) {
+  int o = lane * Eo;
+  const int oend = min(S, o + Eo);
+  while (o < oend) {
+    int l = 0, h = Kp / g - 1;
+    while (l < h) {
+      const int mid = (l + h + 1) >> 1;
+      const bool c = s_off[mid * g] <= o;
+      l = c ? mid : l;
+      h = c ? h : mid - 1;
+    }
+    const int gs = s_off[l * g], gm = s_off[l * g + g / 2], ge = s_off[l * g + g];
+    const int n1 = gm - gs, n2 = ge - gm;
+    const int d = o - gs;
+    // element addresses of the two halves
+    int b1, b2, h1 = 0, h2 = 0;
+    if (FROM_WIN) {
+      b1 = (l * g) * F;  // g == 2: runs 2l and 2l+1
+      b2 = b1 + F;
+      h1 = s_head[l * g];
+      h2 = s_head[l * g + 1];
+    } else {
+      b1 = gs;
+      b2 = gm;
+    }
+    const int msk = F - 1;
+    auto adr1 = [&](int i) { return FROM_WIN ? b1 + ((h1 + i) & msk) : b1 + i; };
+    auto adr2 = [&](int i) { return FROM_WIN ? b2 + ((h2 + i) & msk) : b2 + i; };
+    int plo = d - n2 > 0 ? d - n2 : 0;
+    int phi = d < n1 ? d : n1;
+    while (plo < phi) {
+      const int mid = (plo + phi) >> 1;
+      // A1[mid] is among the first d outputs iff A1[mid] <= A2[d-1-mid] (left run first on ties)
+      const bool c = !lt(lget<E>(src, WM_CAP, adr2(d - 1 - mid)), lget<E>(src, WM_CAP, adr1(mid)));
+      plo = c ? mid + 1 : plo;
+      phi = c ? phi : mid;
+    }
+    int i = plo, jj = d - plo;
+    const int cntm = min(oend, ge) - o;
+    E va = lget<E>(src, WM_CAP, adr1(i));
+    E vb = lget<E>(src, WM_CAP, adr2(jj));
+    for (int e = 0; e < cntm; ++e) {
+      const bool ta = (jj >= n2) | ((i < n1) & !lt(vb, va));
+      lput<E>(dst, WM_CAP, o + e, ta ? va : vb);
+      i += ta ? 1 : 0;
+      jj += ta ? 0 : 1;
+      const E nv = lget<E>(src, WM_CAP, ta ? adr1(i) : adr2(jj));
+      va = ta ? nv : va;
+      vb = ta ? vb : nv;
+    }
+    o += cntm;
+  }
+}
+
 template <int DT, bool CONV_OUT, int WIN>
 __global__ __launch_bounds__(WM_WAVES* WAVE) void k_wmerge(RoundArgs a) {
   using E = typename DType<DT>::E;
@@ -129,19 +191,40 @@ __global__ __launch_bounds__(WM_WAVES* WAVE) void k_wmerge(RoundArgs a) {
       // all loads in flight first; remember which slots got one
       E v[CHUNKS];
       uint32_t vmask = 0;
+      // Loads are unconditional (a slot with nothing to fetch reads element 0,
+      // always in bounds) so that no branch join forces a wait between them.
+      if (F >= WAVE) {
+        // a chunk lies inside one list: its state is a wave-uniform readlane
 #pragma unroll
-      for (int u = 0; u < CHUNKS; ++u) {
-        const int slot = u * WAVE + lane;
-        const int l = slot / F;  // active list of this slot
-        const int sj = slot & (F - 1);
-        const int hl = __shfl(head, l, WAVE);
-        const int cl = __shfl(cnt, l, WAVE);
-        const int wl = __shfl(want, l, WAVE);
-        const int64_t cu = __shfl(cur, l, WAVE);
-        const int i = (sj - hl) & (F - 1);  // logical index of this slot
-        if (l < Ka && i >= cl && i < wl) {
-          v[u] = gload<DT, false>(a.in_keys, a.in_vals, cu + i);
-          vmask |= 1u << u;
+        for (int u = 0; u < CHUNKS; ++u) {
+          const int slot = u * WAVE + lane;
+          const int lr = (u * WAVE) / F;
+          const int l = lr < WAVE ? lr : WAVE - 1;
+          const int sj = slot & (F - 1);
+          const int hl = __builtin_amdgcn_readlane(head, l);
+          const int cl = __builtin_amdgcn_readlane(cnt, l);
+          const int wl = __builtin_amdgcn_readlane(want, l);
+          const int64_t cu = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(cur >> 32), l) << 32) |
+                                       (uint32_t)__builtin_amdgcn_readlane((int)cur, l));
+          const int i = (sj - hl) & (F - 1);
+          const bool ok = lr < Ka && i >= cl && i < wl;
+          v[u] = gload<DT, false>(a.in_keys, a.in_vals, ok ? cu + i : 0);
+          vmask |= ok ? (1u << u) : 0u;
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < CHUNKS; ++u) {
+          const int slot = u * WAVE + lane;
+          const int l = slot / F;  // active list of this slot (< 64: F >= 16)
+          const int sj = slot & (F - 1);
+          const int hl = __shfl(head, l, WAVE);
+          const int cl = __shfl(cnt, l, WAVE);
+          const int wl = __shfl(want, l, WAVE);
+          const int64_t cu = __shfl(cur, l, WAVE);
+          const int i = (sj - hl) & (F - 1);  // logical index of this slot
+          const bool ok = l < Ka && i >= cl && i < wl;
+          v[u] = gload<DT, false>(a.in_keys, a.in_vals, ok ? cu + i : 0);
+          vmask |= ok ? (1u << u) : 0u;
         }
       }
 #pragma unroll
@@ -258,72 +341,40 @@ __global__ __launch_bounds__(WM_WAVES* WAVE) void k_wmerge(RoundArgs a) {
     if (lane == WAVE - 1) s_off[WAVE] = S;
     wave_lds_sync();
 
-    // ---- gather into bufA: slot-parallel over the windows
-#pragma unroll
-    for (int u = 0; u < CHUNKS; ++u) {
-      const int slot = u * WAVE + lane;
-      const int l = slot / F;
-      const int i = slot & (F - 1);  // logical index within list l's prefix
-      if (l < Ka) {
-        const int tl = s_take[l];
-        if (i < tl) {
-          const int hl = s_head[l];
-          lput<E>(bufA, WIN, s_off[l] + i, lget<E>(win, WIN, l * F + ((hl + i) & (F - 1))));
-        }
-      }
-    }
-    wave_lds_sync();
-
-    // ---- stable pairwise merge passes over Kp runs at s_off[]
-    uint32_t* src = bufA;
-    uint32_t* dst = bufB;
+    // ---- stable pairwise merge passes over Kp runs at s_off[]; the first
+    // pass reads the runs straight out of the circular windows
     const int Eo = (S + WAVE - 1) / WAVE;
-    for (int g = 2; g <= Kp; g <<= 1) {
-      int o = lane * Eo;
-      const int oend = min(S, o + Eo);
-      while (o < oend) {
-        int l = 0, h = Kp / g - 1;
-        while (l < h) {
-          const int mid = (l + h + 1) >> 1;
-          const bool c = s_off[mid * g] <= o;
-          l = c ? mid : l;
-          h = c ? h : mid - 1;
+    if (Kp == 1) {
+      // one run: it is window 0 (list 0's, held by lane 0), in order from its head
+      const int h0 = __builtin_amdgcn_readlane(head, 0);
+      for (int o = lane; o < S; o += WAVE)
+        gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + o, lget<E>(win, WIN, (h0 + o) & (F - 1)));
+    } else {
+      // gather the taken prefixes into bufA (run r at s_off[r])
+#pragma unroll
+      for (int u = 0; u < CHUNKS; ++u) {
+        const int slot = u * WAVE + lane;
+        const int l = slot / F;
+        const int i = slot & (F - 1);  // logical index within list l's prefix
+        if (l < Ka) {
+          const int tl = s_take[l];
+          if (i < tl) lput<E>(bufA, WIN, s_off[l] + i, lget<E>(win, WIN, l * F + ((s_head[l] + i) & (F - 1))));
         }
-        const int gs = s_off[l * g], gm = s_off[l * g + g / 2], ge = s_off[l * g + g];
-        const int n1 = gm - gs, n2 = ge - gm;
-        const int d = o - gs;
-        int plo = d - n2 > 0 ? d - n2 : 0;
-        int phi = d < n1 ? d : n1;
-        while (plo < phi) {
-          const int mid = (plo + phi) >> 1;
-          const bool c = !lt(lget<E>(src, WIN, gm + d - 1 - mid), lget<E>(src, WIN, gs + mid));
-          plo = c ? mid + 1 : plo;
-          phi = c ? phi : mid;
-        }
-        int i = plo, jj = d - plo;
-        const int cntm = min(oend, ge) - o;
-        // reads one past a run end stay inside this merger's LDS (index <= WIN)
-        E va = lget<E>(src, WIN, gs + i);
-        E vb = lget<E>(src, WIN, gm + jj);
-        for (int e = 0; e < cntm; ++e) {
-          const bool ta = (jj >= n2) | ((i < n1) & !lt(vb, va));
-          lput<E>(dst, WIN, o + e, ta ? va : vb);
-          i += ta ? 1 : 0;
-          jj += ta ? 0 : 1;
-          const E nv = lget<E>(src, WIN, ta ? gs + i : gm + jj);
-          va = ta ? nv : va;
-          vb = ta ? vb : nv;
-        }
-        o += cntm;
       }
       wave_lds_sync();
-      uint32_t* t = src;
-      src = dst;
-      dst = t;
+      uint32_t* src = bufA;
+      uint32_t* dst = bufB;
+      for (int g = 2; g <= Kp; g <<= 1) {
+        merge_pass<E, false, WIN>(src, dst, g, Kp, S, Eo, lane, s_off, s_head, F);
+        wave_lds_sync();
+        uint32_t* t = src;
+        src = dst;
+        dst = t;
+      }
+      for (int o = lane; o < S; o += WAVE) gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + o, lget<E>(src, WIN, o));
     }
 
-    // ---- store and advance
-    for (int o = lane; o < S; o += WAVE) gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + o, lget<E>(src, WIN, o));
+    // ---- advance
     obase += S;
     cur += sk;
     head = (head + sk) & (F - 1);

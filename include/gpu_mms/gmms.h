@@ -50,15 +50,21 @@ typedef enum gmms_dtype {
 } gmms_dtype;
 
 /* Run-time knobs. Replaces the compile-time macros of src/params.h:18-38:
- * K (branching factor, a power of two, 2..64), PIPELINE (one wavefront per
- * heap level; 0 is accepted and means the same pipelined kernel), and
- * BLOCKS (= threadBlocks, the merger count when a round has few tasks).
- * Zero fields take the defaults of gmms_get_default_params. */
+ * K (branching factor, a power of two, 2..64), PIPELINE (which K-way merge
+ * kernel runs a round) and BLOCKS (= threadBlocks, the merger count when a
+ * round has few tasks). Zero fields take the defaults of
+ * gmms_get_default_params. */
+enum {
+  GMMS_MERGE_WAVEFRONT = 0, /* tile merge, one merger per wavefront (default) */
+  GMMS_MERGE_MINBLOCKHEAP = 1, /* the reference's PIPELINE minBlockHeap:
+                                  one wavefront per heap level            */
+  GMMS_MERGE_WORKGROUP = 2  /* tile merge, one merger per workgroup        */
+};
 typedef struct gmms_params {
-  int32_t kway;      /* K: lists merged per round; 0: auto = the largest power
-                        of two <= 64 whose heap fits LDS (64 for 4- and
-                        8-byte keys, 32 for pairs)                           */
-  int32_t pipeline;  /* 1: one wave per heap level (the only kernel)          */
+  int32_t kway;      /* K: lists merged per round; 0: auto (16 for the
+                        wavefront merger; otherwise the largest power of two
+                        <= 64 whose LDS state fits)                          */
+  int32_t pipeline;  /* merge kernel, GMMS_MERGE_*                           */
   int32_t mergers;   /* target mergers per round when tasks are few; 0: auto */
   int32_t prefetch;  /* blocks prefetched per list into LDS (1..8)           */
   int32_t slack;     /* partition tolerance in elements; 0: auto             */
@@ -72,7 +78,7 @@ const char* gmms_status_string(int status);
 /* The HIP call and error behind the calling thread's last GMMS_ERROR_HIP
  * ("" if none). */
 const char* gmms_last_error(void);
-/* Fill *p with the defaults (K auto, PIPELINE on, auto mergers). */
+/* Fill *p with the defaults (K auto, wavefront merger, auto mergers). */
 int gmms_get_default_params(gmms_params* p);
 /* Bytes per key and per value (0 for key-only dtypes) of a dtype. */
 int gmms_element_size(int dtype, size_t* key_bytes, size_t* value_bytes);

@@ -59,11 +59,7 @@ T* multimergesort(T* input, T* output, T* h_data, int threadBlocks, int64_t N) {
   gmms_params p;
   gmms_get_default_params(&p);
   p.kway = K;
-#ifdef PIPELINE
-  p.pipeline = 1;
-#else
-  p.pipeline = 0;
-#endif
+  p.pipeline = MERGE_KERNEL;
   p.mergers = threadBlocks > 0 ? threadBlocks : BLOCKS;
   const int st = gmms_multimergesort_params(gpu_mms::dtype_of<T>::value, input, output, h_data, &p, N, &result);
   gpu_mms::last_status_storage(st);

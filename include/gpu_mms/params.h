@@ -4,9 +4,12 @@
  * value can also be chosen at run time through the C ABI (gmms.h).
  *
  *   K         lists merged per round (reference: 64)
- *   PL        log2(K): heap levels = wavefronts per merger when PIPELINE is on
- *   PIPELINE  one wavefront per heap level (the reference's pipelined heap);
- *             the MI355X build has only the pipelined kernel
+ *   PL        log2(K): heap levels = wavefronts per merger of the heap kernel
+ *   PIPELINE  defined as in the reference (the minBlockHeap kernel is always
+ *             the pipelined one: one wavefront per heap level)
+ *   MERGE_KERNEL  which K-way merge runs the rounds: 0 tile merge with one
+ *             merger per wavefront (default, fastest), 1 the pipelined
+ *             minBlockHeap, 2 tile merge with one merger per workgroup
  *   RANGE     range of the harness's random keys, rand() % RANGE
  *   BLOCKS    mergers per round when a round has few tasks (reference: 482;
  *             here 0 = two per compute unit, 512 on MI355X)
@@ -17,11 +20,14 @@
 #ifndef GPU_MMS_PARAMS_H
 #define GPU_MMS_PARAMS_H
 
+#ifndef MERGE_KERNEL
+#define MERGE_KERNEL 0
+#endif
 #ifndef K
-#define K 64
+#define K 16
 #endif
 #ifndef PL
-#define PL 6
+#define PL 4
 #endif
 #ifndef NO_PIPELINE
 #define PIPELINE

@@ -31,7 +31,7 @@ def test_version_and_status_strings(gmms):
 
 def test_default_params(gmms):
     p = gmms.default_params()
-    assert (p.kway, p.pipeline, p.mergers) == (0, 1, 0)  # K=0: auto (64, or 32 for pairs)
+    assert (p.kway, p.pipeline, p.mergers) == (0, 0, 0)  # K auto, wavefront merger, auto mergers
     assert 1 <= p.prefetch <= 8
 
 

@@ -1,14 +1,16 @@
-"""GPU parity of the two tile merge kernels -- pipeline=0 (one merger per
-wavefront) and pipeline=2 (one merger per workgroup) -- against the oracle,
-bit-exact: the same cases as the heap kernel plus presorted and clustered
-inputs that drive their galloping run copies."""
+"""GPU parity of every K-way merge kernel -- pipeline=0 (tile merge, one
+merger per wavefront; the default), 1 (the pipelined minBlockHeap) and 2
+(tile merge, one merger per workgroup) -- against the oracle, bit-exact:
+sizes, branching factors, merger counts, duplicate-heavy, presorted and
+clustered inputs (which drive the tile merges' galloping run copies), float
+total order, 64-bit keys and (key, value) pairs."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(params=[0, 2], ids=["wave", "workgroup"])
+@pytest.fixture(params=[0, 1, 2], ids=["wave", "heap", "workgroup"])
 def kern(request):
     return request.param
 
