@@ -43,25 +43,32 @@ def parse():
     ap.add_argument("--K", type=int, default=0, help="lists per merge round (0: auto)")
     ap.add_argument("--mergers", type=int, default=0)
     ap.add_argument("--prefetch", type=int, default=0)
+    ap.add_argument("--kernel", type=int, default=None,
+                    help="merge kernel: 0 wavefront tile (default), 1 minBlockHeap, 2 workgroup tile")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     return ap.parse_args()
 
 
-def traffic_from_profiles():
-    """HBM bytes per merge launch measured by rocprofv3 (profiles/merge_traffic.json)."""
+def traffic_from_profiles(kernel: int, K: int, n: int):
+    """HBM bytes per merge launch of this workload, measured by rocprofv3
+    FETCH_SIZE/WRITE_SIZE and calibrated with the copy kernel
+    (profiles/merge_traffic.json); None when no profile matches."""
     p = os.path.join(ROOT, "profiles", "merge_traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
-        return json.load(f).get("hbm_bytes_per_merge_launch")
+        d = json.load(f)
+    if d.get("pipeline") != kernel or d.get("K") != K or d.get("n") != n:
+        return None
+    return d.get("hbm_bytes_per_merge_launch")
 
 
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design note "
-              "(see DESIGN.md), --gpus must be 1", file=sys.stderr)
+        print("bench.py: this build sorts on one GPU; runs on more than one GPU are not part of it "
+              "(--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -81,10 +88,13 @@ def main():
     consts = [0] + [int(c) for c in rng.integers(1, 2**31 - 1, total - 1)]
     inputs = [base if c == 0 else torch.bitwise_xor(base, c) for c in consts]
     alt = torch.empty_like(base)
-    params = gmms.make_params(K=args.K, mergers=args.mergers, prefetch=args.prefetch)
+    params = gmms.make_params(K=args.K, mergers=args.mergers, prefetch=args.prefetch, pipeline=args.kernel)
     ws = gmms.Workspace()
     stream = torch.cuda.current_stream()
-    K = args.K or 64
+    resolved = gmms.resolve_params(gmms.INT32, params)
+    K = resolved.kway
+    kernel_name = {0: "k_wmerge (wavefront tile merge)", 1: "k_merge (pipelined minBlockHeap)",
+                   2: "k_tmerge (workgroup tile merge)"}[resolved.pipeline]
     rounds = gmms.num_rounds(n, K)
     # keep an untouched copy of input 0 for the bit-exact check
     check_src = base.clone()
@@ -178,12 +188,13 @@ def main():
         "dtype": "int32",
         "data": f"synthetic: mt19937_64 seed {seed:#x}, key=(int32)(u64>>32); step i XORs a constant",
         "config": {"workload": "plain sort, N=2^28 int32 uniform over the full int32 range",
-                   "n": n, "K": K, "rounds": rounds, "mergers": args.mergers or "auto"},
+                   "n": n, "K": K, "rounds": rounds, "merge_kernel": resolved.pipeline,
+                   "mergers": resolved.mergers},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2) if achieved else None,
                      "peak": PUBLISHED_HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / PUBLISHED_HBM_PEAK_GBPS, 4) if achieved else None,
-                     "traffic": traffic_from_profiles(),
-                     "kernel": "k_merge (one K-way merge round)", "algorithmic_bytes_per_launch": merge_bytes,
+                     "traffic": traffic_from_profiles(resolved.pipeline, K, n),
+                     "kernel": kernel_name, "algorithmic_bytes_per_launch": merge_bytes,
                      "avg_launch_ms": round(merge_avg, 4)},
         "cpu_baseline": cpu,
         "bit_exact_vs_std_sort": bit_exact,

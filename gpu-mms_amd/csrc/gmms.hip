@@ -458,6 +458,21 @@ int gmms_get_default_params(gmms_params* p) {
   return GMMS_SUCCESS;
 }
 
+int gmms_resolve_params(int dtype, const gmms_params* in, gmms_params* out) {
+  if (!valid_dtype(dtype) || !out) return GMMS_ERROR_INVALID_VALUE;
+  Resolved r;
+  int st = resolve(dtype, in, &r, true);
+  if (st) return st;
+  gmms_get_default_params(out);
+  if (in) *out = *in;
+  out->kway = r.K;
+  out->pipeline = r.kernel;
+  out->mergers = r.mergers;
+  out->prefetch = r.D;
+  out->slack = (int32_t)r.slack;
+  return GMMS_SUCCESS;
+}
+
 int gmms_element_size(int dtype, size_t* kb, size_t* vb) {
   if (!valid_dtype(dtype) || !kb || !vb) return GMMS_ERROR_INVALID_VALUE;
   const int w = elem_words(dtype);

@@ -96,6 +96,7 @@ def lib() -> ctypes.CDLL:
         "gmms_last_error": (ctypes.c_char_p, []),
         "gmms_multimergesort_params": (i32, [i32, vp, vp, vp, P, i64, ctypes.POINTER(vp)]),
         "gmms_get_default_params": (i32, [P]),
+        "gmms_resolve_params": (i32, [i32, P, P]),
         "gmms_element_size": (i32, [i32, ctypes.POINTER(sz), ctypes.POINTER(sz)]),
         "gmms_num_rounds": (i32, [i64, i32, ctypes.POINTER(i32)]),
         "gmms_workspace_size": (i32, [i32, i64, P, ctypes.POINTER(sz)]),
@@ -159,6 +160,13 @@ def make_params(K: int = 0, mergers: int = 0, prefetch: int = 0, slack: int = 0,
     if slack:
         p.slack = slack
     return p
+
+
+def resolve_params(dtype: int, params: Optional[Params] = None) -> Params:
+    """The K, merge kernel and merger target a sort of `dtype` runs with."""
+    out = Params()
+    _check(lib().gmms_resolve_params(dtype, _pp(params), ctypes.byref(out)), "gmms_resolve_params")
+    return out
 
 
 def num_rounds(n: int, K: int = 64) -> int:

@@ -80,6 +80,9 @@ const char* gmms_status_string(int status);
 const char* gmms_last_error(void);
 /* Fill *p with the defaults (K auto, wavefront merger, auto mergers). */
 int gmms_get_default_params(gmms_params* p);
+/* The knobs a sort of `dtype` would actually run with: zero/auto fields of
+ * *in (NULL: defaults) resolved (K, merge kernel, merger target). */
+int gmms_resolve_params(int dtype, const gmms_params* in, gmms_params* out);
 /* Bytes per key and per value (0 for key-only dtypes) of a dtype. */
 int gmms_element_size(int dtype, size_t* key_bytes, size_t* value_bytes);
 /* Number of merge rounds for n elements: iterations of the reference's
