@@ -61,7 +61,52 @@ struct RoundArgs {
   int64_t slack;  // partition tolerance (elements discarded at most)
   int64_t* cuts;  // [(task*m + j) * K + k]
   int64_t* skips; // [task*m + j]
+  // batched segments: merger descriptors instead of the uniform task grid
+  const struct MergerInfo* items;  // nullptr for a plain sort
+  int64_t n_items;
+  unsigned int* queue;             // work-queue counter of a persistent launch
 };
+
+// Geometry of one merger: task base (element index of its list 0), task size
+// T, list length L, mergers per task m, its index j, and the cut row of its
+// task's merger 0 (merger j reads cut rows cut_row+j and cut_row+j+1).
+struct MergerInfo {
+  int64_t tbase;
+  int64_t T;
+  int64_t L;
+  int64_t cut_row;
+  int32_t m;
+  int32_t j;
+};
+
+__device__ __forceinline__ MergerInfo plain_merger(const RoundArgs& a, int64_t merger) {
+  MergerInfo mi;
+  const int64_t task = merger / a.m;
+  mi.j = (int32_t)(merger % a.m);
+  mi.m = a.m;
+  mi.L = a.L;
+  mi.tbase = task * (int64_t)a.K * a.L;
+  const int64_t rest = a.n - mi.tbase;
+  const int64_t full = (int64_t)a.K * a.L;
+  mi.T = rest < full ? rest : full;
+  mi.cut_row = task * a.m;
+  return mi;
+}
+__device__ __forceinline__ MergerInfo load_merger(const MergerInfo* p) {
+  MergerInfo mi;
+  mi.tbase = p->tbase;
+  mi.T = p->T;
+  mi.L = p->L;
+  mi.cut_row = p->cut_row;
+  mi.m = p->m;
+  mi.j = p->j;
+  return mi;
+}
+__device__ __forceinline__ int64_t mlist_len(const MergerInfo& mi, int k) {
+  int64_t l = mi.T - (int64_t)k * mi.L;
+  if (l < 0) l = 0;
+  return l < mi.L ? l : mi.L;
+}
 
 __device__ __forceinline__ int64_t task_size(const RoundArgs& a, int64_t task) {
   const int64_t tb = task * (int64_t)a.K * a.L;
@@ -89,20 +134,17 @@ __device__ __forceinline__ int64_t wave_sum64(int64_t x) {
 // -------------------------------------------------------------- partition
 constexpr int PART_WAVES = 4;
 
+// Consistent cut of merger mi.j >= 1 of its task: writes cut row
+// mi.cut_row + mi.j (K offsets) and its skip.
 template <int DT>
-__global__ __launch_bounds__(PART_WAVES * WAVE) void k_partition(RoundArgs a) {
+__device__ __forceinline__ void partition_one(const RoundArgs& a, const MergerInfo& mi, int lane) {
   using E = typename DType<DT>::E;
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int64_t wid = (int64_t)blockIdx.x * PART_WAVES + threadIdx.x / WAVE;
-  const int64_t per_task = a.m - 1;
-  if (wid >= a.tasks * per_task) return;
-  const int64_t task = wid / per_task;
-  const int j = 1 + (int)(wid % per_task);
-  const int64_t T = task_size(a, task);
-  const int64_t r = (int64_t)j * T / a.m;
+  const int j = mi.j;
+  const int64_t T = mi.T;
+  const int64_t r = (int64_t)j * T / mi.m;
   const bool inl = lane < a.K;
-  const int64_t nk = inl ? list_len(a, T, lane) : 0;
-  const int64_t lbase = task * (int64_t)a.K * a.L + (int64_t)lane * a.L;
+  const int64_t nk = inl ? mlist_len(mi, lane) : 0;
+  const int64_t lbase = mi.tbase + (int64_t)lane * mi.L;
 
   int64_t lo = 0, hi = nk, Rlo = 0, Rhi = T;
   for (int it = 0; it < 256 && r - Rlo > a.slack; ++it) {
@@ -172,9 +214,27 @@ __global__ __launch_bounds__(PART_WAVES * WAVE) void k_partition(RoundArgs a) {
       Rhi = R;
     }
   }
-  const int64_t b = task * a.m + j;
+  const int64_t b = mi.cut_row + j;
   if (inl) a.cuts[b * a.K + lane] = lo;
   if (lane == 0) a.skips[b] = r - Rlo;
+}
+
+// One wavefront per merger j >= 1: of the uniform task grid (plain sort), or
+// of the descriptor list (batched segments; descriptors with j == 0 are not
+// listed).
+template <int DT>
+__global__ __launch_bounds__(PART_WAVES * WAVE) void k_partition(RoundArgs a) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int64_t wid = (int64_t)blockIdx.x * PART_WAVES + threadIdx.x / WAVE;
+  if (a.items) {
+    if (wid >= a.n_items) return;
+    partition_one<DT>(a, load_merger(a.items + wid), lane);
+    return;
+  }
+  const int64_t per_task = a.m - 1;
+  if (wid >= a.tasks * per_task) return;
+  const int64_t task = wid / per_task;
+  partition_one<DT>(a, plain_merger(a, task * a.m + 1 + wid % per_task), lane);
 }
 
 // ------------------------------------------------------------------ merge

@@ -120,17 +120,12 @@ __device__ __forceinline__ void merge_pass(const uint32_t* src, uint32_t* dst, i
   }
 }
 
+// One merger, executed by one wavefront using the LDS region `my`.
 template <int DT, bool CONV_OUT, int WIN>
-__global__ __launch_bounds__(WM_WAVES* WAVE) void k_wmerge(RoundArgs a) {
+__device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo& mi, int lane, uint32_t* my) {
   using E = typename DType<DT>::E;
   constexpr int W = E::W;
   constexpr int CHUNKS = WIN / WAVE;
-  extern __shared__ uint32_t lds[];
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int wv = threadIdx.x / WAVE;
-  const int64_t merger = (int64_t)blockIdx.x * WM_WAVES + wv;
-  if (merger >= a.tasks * a.m) return;  // whole wavefront leaves; no barriers below
-  uint32_t* my = lds + (size_t)wv * wmerge_lds_words_per_wave(W, WIN);
   uint32_t* win = my;                 // [list a][F] circular windows, W planes of WIN
   uint32_t* bufA = win + WIN * W;
   uint32_t* bufB = bufA + WIN * W;
@@ -140,17 +135,15 @@ __global__ __launch_bounds__(WM_WAVES* WAVE) void k_wmerge(RoundArgs a) {
   int* s_off = st + 2 * WAVE;         // WAVE + 1 entries
 
   const int K = a.K;
-  const int64_t task = merger / a.m;
-  const int j = (int)(merger % a.m);
-  const int64_t T = task_size(a, task);
-  const int64_t tbase = task * (int64_t)K * a.L;
+  const int j = mi.j;
+  const int64_t tbase = mi.tbase;
 
   // lane k: list k's range [lo, hi) for this merger
   int64_t lo = 0, hi = 0;
   if (lane < K) {
-    const int64_t nk = list_len(a, T, lane);
-    lo = j == 0 ? 0 : a.cuts[merger * K + lane];
-    hi = j == a.m - 1 ? nk : a.cuts[(merger + 1) * K + lane];
+    const int64_t nk = mlist_len(mi, lane);
+    lo = j == 0 ? 0 : a.cuts[(mi.cut_row + j) * K + lane];
+    hi = j == mi.m - 1 ? nk : a.cuts[(mi.cut_row + j + 1) * K + lane];
   }
   int64_t obase = tbase + wave_sum64(lo);
   // compact the non-empty lists, keeping list order
@@ -169,7 +162,7 @@ __global__ __launch_bounds__(WM_WAVES* WAVE) void k_wmerge(RoundArgs a) {
       if (sl == lane) src_lane = l;
     }
   }
-  const int64_t lb = tbase + (int64_t)src_lane * a.L;
+  const int64_t lb = tbase + (int64_t)src_lane * mi.L;
   int64_t cur = lb + __shfl(lo, src_lane, WAVE);
   int64_t endp = lb + __shfl(hi, src_lane, WAVE);
   const bool live = lane < Ka;
@@ -381,6 +374,45 @@ __global__ __launch_bounds__(WM_WAVES* WAVE) void k_wmerge(RoundArgs a) {
     cnt = want - sk;
     wave_lds_sync();
   }
+}
+
+// Plain sort: one merger per wavefront over the uniform task grid.
+template <int DT, bool CONV_OUT, int WIN>
+__global__ __launch_bounds__(WM_WAVES* WAVE) void k_wmerge(RoundArgs a) {
+  using E = typename DType<DT>::E;
+  extern __shared__ uint32_t lds[];
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wv = threadIdx.x / WAVE;
+  const int64_t merger = (int64_t)blockIdx.x * WM_WAVES + wv;
+  if (merger >= a.tasks * a.m) return;  // whole wavefront leaves; no barriers in this kernel
+  uint32_t* my = lds + (size_t)wv * wmerge_lds_words_per_wave(E::W, WIN);
+  wmerge_one<DT, CONV_OUT, WIN>(a, plain_merger(a, merger), lane, my);
+}
+
+// Batched segments: a persistent launch; every wavefront takes merger
+// descriptors from a.items through the a.queue counter until the list is
+// exhausted (every wavefront reaches that exit). The counter starts at zero
+// (k_queue_reset before the launch).
+template <int DT, bool CONV_OUT, int WIN>
+__global__ __launch_bounds__(WM_WAVES* WAVE) void k_wmerge_persistent(RoundArgs a) {
+  using E = typename DType<DT>::E;
+  extern __shared__ uint32_t lds[];
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wv = threadIdx.x / WAVE;
+  uint32_t* my = lds + (size_t)wv * wmerge_lds_words_per_wave(E::W, WIN);
+  for (;;) {
+    unsigned int it = 0;
+    if (lane == 0) it = atomicAdd(a.queue, 1u);
+    it = (unsigned int)__shfl((int)it, 0, WAVE);
+    if ((int64_t)it >= a.n_items) break;
+    wmerge_one<DT, CONV_OUT, WIN>(a, load_merger(a.items + it), lane, my);
+    wave_lds_sync();
+  }
+}
+
+__global__ void k_queue_reset(unsigned int* q, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) q[i] = 0u;
 }
 
 }  // namespace gmms
