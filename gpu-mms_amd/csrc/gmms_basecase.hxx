@@ -155,4 +155,62 @@ __global__ __launch_bounds__(BASE_WAVES * WAVE) void k_basecase(void* keys, uint
   }
 }
 
+// Batched segments: tile descriptors instead of one array. A tile holds
+// `cnt` (1..TILE) elements of one segment at `off`; it is read from buffer 0
+// in the caller's bits and written, sorted, to buffer (flags & 1), in the
+// caller's bits when (flags & 2) (a segment of one tile has no merge round).
+struct TileDesc {
+  int64_t off;
+  int32_t cnt;
+  int32_t flags;
+};
+
+template <int DT>
+__global__ __launch_bounds__(BASE_WAVES* WAVE) void k_basecase_seg(void* k0, uint32_t* v0, void* k1, uint32_t* v1,
+                                                                    const TileDesc* tiles, int64_t n_tiles) {
+  using E = typename DType<DT>::E;
+  __shared__ uint32_t xp[BASE_WAVES * E::W * XPOSE_WORDS];
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wave = threadIdx.x / WAVE;
+  const int64_t tile = (int64_t)blockIdx.x * BASE_WAVES + wave;
+  if (tile >= n_tiles) return;  // per-wave work only: no workgroup barrier below
+  const TileDesc td = tiles[tile];
+  const int64_t base = td.off;
+  const int cnt = td.cnt;
+
+  E v[TPL];
+#pragma unroll
+  for (int r = 0; r < TPL; ++r) {
+    const int idx = r * WAVE + lane;
+    v[r] = idx < cnt ? gload<DT, true>(k0, v0, base + idx) : Limits<E>::hi();
+  }
+
+  sort_tile(v, lane);
+
+  uint32_t* my = xp + wave * E::W * XPOSE_WORDS;
+#pragma unroll
+  for (int w = 0; w < E::W; ++w) {
+#pragma unroll
+    for (int r = 0; r < TPL; ++r) my[w * XPOSE_WORDS + lane * XPOSE_STRIDE + r] = word(v[r], w);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  void* dk = (td.flags & 1) ? k1 : k0;
+  uint32_t* dv = (td.flags & 1) ? v1 : v0;
+  const bool conv = (td.flags & 2) != 0;
+#pragma unroll
+  for (int r = 0; r < TPL; ++r) {
+    const int e = r * WAVE + lane;
+    const int slot = (e / TPL) * XPOSE_STRIDE + (e % TPL);
+    uint32_t w[E::W];
+#pragma unroll
+    for (int q = 0; q < E::W; ++q) w[q] = my[q * XPOSE_WORDS + slot];
+    if (e < cnt) {
+      if (conv) gstore<DT, true>(dk, dv, base + e, from_words<E>(w));
+      else gstore<DT, false>(dk, dv, base + e, from_words<E>(w));
+    }
+  }
+}
+
 }  // namespace gmms

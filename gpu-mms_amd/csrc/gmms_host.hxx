@@ -1,0 +1,211 @@
+// Host-side helpers shared by the translation units of libgmms.so: parameter
+// resolution, launch geometry, error text, opt-in stage timing and the dtype
+// dispatch. Internal; the public interface is include/gpu_mms/gmms.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/gpu_mms/gmms.h"
+#include "gmms_elem.hxx"
+#include "gmms_merge.hxx"
+#include "gmms_tmerge.hxx"
+#include "gmms_wmerge.hxx"
+
+namespace gmms_host {
+using namespace gmms;
+
+struct Resolved {
+  int K, PL, D, mergers;
+  int64_t slack;
+  int kernel;  // GMMS_MERGE_* : wavefront tile, pipelined heap, workgroup tile
+};
+
+enum { GMMS_MERGE_WAVE = 0, GMMS_MERGE_HEAP = 1, GMMS_MERGE_TILE = 2 };
+
+// wavefront-merger window budget (elements): 13 KB of LDS per wavefront
+constexpr int wm_win(int W) { return W == 1 ? 1024 : 512; }
+
+constexpr int TM_F = 64;  // tile-merge window per list
+
+inline int ilog2(int x) {
+  int r = 0;
+  while ((1 << r) < x) ++r;
+  return r;
+}
+
+inline int cu_count() {
+  static int cached = 0;
+  static std::mutex mu;
+  std::lock_guard<std::mutex> g(mu);
+  if (cached) return cached;
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
+  cached = cus;
+  return cached;
+}
+
+inline int elem_words(int dtype) {
+  switch (dtype) {
+    case GMMS_INT32: case GMMS_UINT32: case GMMS_FLOAT32: return 1;
+    case GMMS_INT64: case GMMS_UINT64: case GMMS_FLOAT64: return 2;
+    case GMMS_PAIR_U64_U32: return 3;
+    default: return 0;
+  }
+}
+inline bool valid_dtype(int dtype) { return elem_words(dtype) > 0; }
+inline bool has_vals(int dtype) { return dtype == GMMS_PAIR_U64_U32; }
+
+constexpr size_t LDS_LIMIT = 160 * 1024;  // gfx950 LDS per workgroup
+
+// K = 0 means: the largest power of two <= 64 whose heap + rings fit in LDS
+inline int resolve(int dtype, const gmms_params* p, Resolved* r, bool need_device) {
+  gmms_params d;
+  gmms_get_default_params(&d);
+  const gmms_params& q = p ? *p : d;
+  int D = q.prefetch ? q.prefetch : d.prefetch;
+  if (D < 1 || D > 8) return GMMS_ERROR_INVALID_VALUE;
+  const int W = elem_words(dtype);
+  if (q.pipeline < 0 || q.pipeline > 2) return GMMS_ERROR_INVALID_VALUE;
+  const int kern = q.pipeline;
+  auto lds_bytes = [&](int k) -> size_t {
+    if (kern == GMMS_MERGE_HEAP) return merge_lds_words(k, W, D) * sizeof(uint32_t);
+    if (kern == GMMS_MERGE_TILE) return tmerge_lds_words(k, W, TM_F) * sizeof(uint32_t);
+    // wavefront merger: window per list WIN/K must stay >= 16 elements
+    return wm_win(W) / k >= 16 ? WM_WAVES * wmerge_lds_words_per_wave(W, wm_win(W)) * sizeof(uint32_t)
+                               : LDS_LIMIT + 1;
+  };
+  int K = q.kway;
+  if (K == 0) {
+    // wavefront merger: 16 lists (64- or 32-element windows); the others:
+    // the largest power of two that fits LDS
+    K = kern == GMMS_MERGE_WAVE ? 16 : MAX_K;
+    while (K > 2 && lds_bytes(K) > LDS_LIMIT) K >>= 1;
+  }
+  if (K < 2 || K > MAX_K || (K & (K - 1))) return GMMS_ERROR_INVALID_VALUE;
+  if (lds_bytes(K) > LDS_LIMIT) return GMMS_ERROR_UNSUPPORTED;
+  r->kernel = kern;
+  if (q.mergers < 0 || q.slack < 0) return GMMS_ERROR_INVALID_VALUE;
+  r->K = K;
+  r->PL = ilog2(K);
+  r->D = D;
+  // resident mergers per CU: heap 2 (LDS), workgroup tile 3, wavefront tile 12
+  const int per_cu = kern == GMMS_MERGE_HEAP ? 2 : (kern == GMMS_MERGE_TILE ? 3 : 12);
+  r->mergers = q.mergers ? q.mergers : per_cu * (need_device ? cu_count() : 256);
+  r->slack = q.slack;
+  return GMMS_SUCCESS;
+}
+
+// mergers per task for a round
+inline int mergers_per_task(int64_t n, int64_t L, int K, int target) {
+  const int64_t TK = (int64_t)K * L;
+  const int64_t tasks = (n + TK - 1) / TK;
+  if (tasks >= target) return 1;
+  int64_t m = (target + tasks - 1) / tasks;
+  // keep every merger at least a few heap-builds worth of output
+  const int64_t minsz = (int64_t)WAVE * K * 4;
+  const int64_t tsz = std::min<int64_t>(n, TK);
+  m = std::min<int64_t>(m, std::max<int64_t>(1, tsz / minsz));
+  return (int)std::max<int64_t>(1, m);
+}
+
+inline int64_t slack_for(const Resolved& r, int64_t n, int64_t L, int m) {
+  if (r.slack) return r.slack;
+  const int64_t TK = (int64_t)r.K * L;
+  const int64_t tsz = std::min<int64_t>(n, TK);
+  return std::max<int64_t>(WAVE, tsz / m / 32);
+}
+
+inline size_t ws_bytes_for(int64_t n, const Resolved& r) {
+  size_t need = 0;
+  for (int64_t L = TILE; L < n; L *= r.K) {
+    const int64_t TK = (int64_t)r.K * L;
+    const int64_t tasks = (n + TK - 1) / TK;
+    const int m = mergers_per_task(n, L, r.K, r.mergers);
+    if (m > 1) {
+      const size_t b = (size_t)(tasks * m);
+      need = std::max(need, b * (size_t)(r.K + 1) * sizeof(int64_t));
+    }
+  }
+  return need;
+}
+
+extern thread_local char g_last_error[256];
+
+inline int hip_fail(hipError_t e, const char* what) {
+  std::snprintf(g_last_error, sizeof(g_last_error), "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
+  return GMMS_ERROR_HIP;
+}
+
+#define HIP_OK(x)                                    \
+  do {                                               \
+    hipError_t e_ = (x);                             \
+    if (e_ != hipSuccess) return hip_fail(e_, #x);   \
+  } while (0)
+
+// Opt-in per-launch timing: HIP events recorded on the launch stream around
+// every kernel, read back by gmms_stage_times().
+struct StageRec {
+  hipEvent_t a, b;
+  int kind;
+  int64_t elems;
+};
+struct StageTimer {
+  std::mutex mu;
+  bool on = false;
+  std::vector<StageRec> recs;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  hipEvent_t get() {
+    if (used == pool.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      pool.push_back(e);
+    }
+    return pool[used++];
+  }
+};
+extern StageTimer g_timer;
+
+struct StageScope {
+  hipEvent_t b = nullptr;
+  hipStream_t s;
+  StageScope(int kind, int64_t elems, hipStream_t st) : s(st) {
+    if (!g_timer.on) return;
+    std::lock_guard<std::mutex> g(g_timer.mu);
+    hipEvent_t a = g_timer.get();
+    b = g_timer.get();
+    if (!a || !b) {
+      b = nullptr;
+      return;
+    }
+    (void)hipEventRecord(a, s);
+    g_timer.recs.push_back({a, b, kind, elems});
+  }
+  ~StageScope() {
+    if (b) (void)hipEventRecord(b, s);
+  }
+};
+
+template <class F>
+inline int dispatch(int dtype, F&& f) {
+  switch (dtype) {
+    case GMMS_INT32: return f(std::integral_constant<int, DT_I32>{});
+    case GMMS_UINT32: return f(std::integral_constant<int, DT_U32>{});
+    case GMMS_FLOAT32: return f(std::integral_constant<int, DT_F32>{});
+    case GMMS_INT64: return f(std::integral_constant<int, DT_I64>{});
+    case GMMS_UINT64: return f(std::integral_constant<int, DT_U64>{});
+    case GMMS_FLOAT64: return f(std::integral_constant<int, DT_F64>{});
+    case GMMS_PAIR_U64_U32: return f(std::integral_constant<int, DT_PAIR>{});
+    default: return GMMS_ERROR_INVALID_VALUE;
+  }
+}
+
+}  // namespace gmms_host

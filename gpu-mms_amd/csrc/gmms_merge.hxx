@@ -59,8 +59,9 @@ struct RoundArgs {
   int m;          // mergers per task
   int D;          // ring depth (blocks prefetched per list)
   int64_t slack;  // partition tolerance (elements discarded at most)
-  int64_t* cuts;  // [(task*m + j) * K + k]
+  int64_t* cuts;  // [(task*m + j) * cut_stride + k]
   int64_t* skips; // [task*m + j]
+  int cut_stride; // entries per cut row: K (plain sort) or MAX_K (batched)
   // batched segments: merger descriptors instead of the uniform task grid
   const struct MergerInfo* items;  // nullptr for a plain sort
   int64_t n_items;
@@ -77,6 +78,8 @@ struct MergerInfo {
   int64_t cut_row;
   int32_t m;
   int32_t j;
+  int32_t K;  // lists of this merger's task
+  int32_t pad_;
 };
 
 __device__ __forceinline__ MergerInfo plain_merger(const RoundArgs& a, int64_t merger) {
@@ -90,6 +93,8 @@ __device__ __forceinline__ MergerInfo plain_merger(const RoundArgs& a, int64_t m
   const int64_t full = (int64_t)a.K * a.L;
   mi.T = rest < full ? rest : full;
   mi.cut_row = task * a.m;
+  mi.K = a.K;
+  mi.pad_ = 0;
   return mi;
 }
 __device__ __forceinline__ MergerInfo load_merger(const MergerInfo* p) {
@@ -100,6 +105,8 @@ __device__ __forceinline__ MergerInfo load_merger(const MergerInfo* p) {
   mi.cut_row = p->cut_row;
   mi.m = p->m;
   mi.j = p->j;
+  mi.K = p->K;
+  mi.pad_ = 0;
   return mi;
 }
 __device__ __forceinline__ int64_t mlist_len(const MergerInfo& mi, int k) {
@@ -142,12 +149,15 @@ __device__ __forceinline__ void partition_one(const RoundArgs& a, const MergerIn
   const int j = mi.j;
   const int64_t T = mi.T;
   const int64_t r = (int64_t)j * T / mi.m;
-  const bool inl = lane < a.K;
+  const int K = mi.K;
+  const bool inl = lane < K;
+  // the plain sort passes one slack per round; batched descriptors size it per task
+  const int64_t slack = a.slack > 0 ? a.slack : max((int64_t)WAVE, T / mi.m / 32);
   const int64_t nk = inl ? mlist_len(mi, lane) : 0;
   const int64_t lbase = mi.tbase + (int64_t)lane * mi.L;
 
   int64_t lo = 0, hi = nk, Rlo = 0, Rhi = T;
-  for (int it = 0; it < 256 && r - Rlo > a.slack; ++it) {
+  for (int it = 0; it < 256 && r - Rlo > slack; ++it) {
     const int64_t Wt = Rhi - Rlo;  // == sum(hi - lo)
     const int64_t w = hi - lo;
     const bool act = w > 0;
@@ -163,7 +173,7 @@ __device__ __forceinline__ void partition_one(const RoundArgs& a, const MergerIn
     const E x = act ? ld_int<DT>(a, lbase + pos) : Limits<E>::hi();
     // rank of this candidate among active candidates, order (element, lane)
     int rank = 0;
-    for (int i = 0; i < a.K; ++i) {
+    for (int i = 0; i < K; ++i) {
       const bool ai = __shfl((int)act, i, WAVE) != 0;
       uint32_t wi[E::W];
 #pragma unroll
@@ -173,7 +183,7 @@ __device__ __forceinline__ void partition_one(const RoundArgs& a, const MergerIn
     }
     // inclusive weight prefix in sorted order
     int64_t pre = 0;
-    for (int i = 0; i < a.K; ++i) {
+    for (int i = 0; i < K; ++i) {
       const int ri = __shfl(rank, i, WAVE);
       const int64_t wi = __shfl(act ? w : (int64_t)0, i, WAVE);
       const bool ai = __shfl((int)act, i, WAVE) != 0;
@@ -215,8 +225,8 @@ __device__ __forceinline__ void partition_one(const RoundArgs& a, const MergerIn
     }
   }
   const int64_t b = mi.cut_row + j;
-  if (inl) a.cuts[b * a.K + lane] = lo;
-  if (lane == 0) a.skips[b] = r - Rlo;
+  if (inl) a.cuts[b * a.cut_stride + lane] = lo;
+  if (lane == 0 && a.skips) a.skips[b] = r - Rlo;
 }
 
 // One wavefront per merger j >= 1: of the uniform task grid (plain sort), or

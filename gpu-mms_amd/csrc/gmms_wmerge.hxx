@@ -134,7 +134,7 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
   int* s_take = st + WAVE;
   int* s_off = st + 2 * WAVE;         // WAVE + 1 entries
 
-  const int K = a.K;
+  const int K = mi.K;
   const int j = mi.j;
   const int64_t tbase = mi.tbase;
 
@@ -142,8 +142,8 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
   int64_t lo = 0, hi = 0;
   if (lane < K) {
     const int64_t nk = mlist_len(mi, lane);
-    lo = j == 0 ? 0 : a.cuts[(mi.cut_row + j) * K + lane];
-    hi = j == mi.m - 1 ? nk : a.cuts[(mi.cut_row + j + 1) * K + lane];
+    lo = j == 0 ? 0 : a.cuts[(mi.cut_row + j) * a.cut_stride + lane];
+    hi = j == mi.m - 1 ? nk : a.cuts[(mi.cut_row + j + 1) * a.cut_stride + lane];
   }
   int64_t obase = tbase + wave_sum64(lo);
   // compact the non-empty lists, keeping list order
@@ -392,7 +392,7 @@ __global__ __launch_bounds__(WM_WAVES* WAVE) void k_wmerge(RoundArgs a) {
 // Batched segments: a persistent launch; every wavefront takes merger
 // descriptors from a.items through the a.queue counter until the list is
 // exhausted (every wavefront reaches that exit). The counter starts at zero
-// (k_queue_reset before the launch).
+// (cleared by the launch before).
 template <int DT, bool CONV_OUT, int WIN>
 __global__ __launch_bounds__(WM_WAVES* WAVE) void k_wmerge_persistent(RoundArgs a) {
   using E = typename DType<DT>::E;
@@ -408,11 +408,6 @@ __global__ __launch_bounds__(WM_WAVES* WAVE) void k_wmerge_persistent(RoundArgs 
     wmerge_one<DT, CONV_OUT, WIN>(a, load_merger(a.items + it), lane, my);
     wave_lds_sync();
   }
-}
-
-__global__ void k_queue_reset(unsigned int* q, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) q[i] = 0u;
 }
 
 }  // namespace gmms

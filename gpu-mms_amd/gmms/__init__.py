@@ -69,6 +69,24 @@ class Params(ctypes.Structure):
     ]
 
 
+class SegInfo(ctypes.Structure):
+    """gmms_seg_info (gmms.h): the shape of a planned batch."""
+
+    _fields_ = [
+        ("n_seg", ctypes.c_int64),
+        ("elements", ctypes.c_int64),
+        ("extent", ctypes.c_int64),
+        ("tiles", ctypes.c_int64),
+        ("merge_items", ctypes.c_int64),
+        ("partition_items", ctypes.c_int64),
+        ("workspace_bytes", ctypes.c_size_t),
+        ("rounds", ctypes.c_int32),
+        ("launches", ctypes.c_int32),
+    ]
+
+
+SEG_CLASSES = 41  # GMMS_SEG_CLASSES
+
 _lib = None
 
 
@@ -110,6 +128,18 @@ def lib() -> ctypes.CDLL:
         "gmms_stage_timing": (i32, [i32]),
         "gmms_copy": (i32, [vp, vp, sz, vp]),
         "gmms_stage_times": (i32, [vp, vp, vp, i32, ctypes.POINTER(i32)]),
+        "gmms_seg_plan_create": (i32, [i32, vp, i64, P, ctypes.POINTER(vp)]),
+        "gmms_seg_plan_info": (i32, [vp, ctypes.POINTER(SegInfo)]),
+        "gmms_seg_plan_segment": (i32, [vp, i64, ctypes.POINTER(i32), ctypes.POINTER(i32)]),
+        "gmms_seg_plan_destroy": (None, [vp]),
+        "gmms_seg_sort": (i32, [vp, vp, vp, vp, vp, vp, sz, vp, ctypes.POINTER(i32)]),
+        "gmms_seg_graph_create": (i32, [vp, vp, vp, vp, vp, vp, sz, ctypes.POINTER(vp)]),
+        "gmms_seg_graph_launch": (i32, [vp, vp, ctypes.POINTER(i32)]),
+        "gmms_seg_graph_destroy": (None, [vp]),
+        "gmms_segmented_sort": (i32, [i32, vp, vp, vp, vp, vp, i64, P, vp, ctypes.POINTER(i32)]),
+        "gmms_seg_tuning_get": (i32, [i32, vp, vp, i32]),
+        "gmms_seg_tuning_set": (i32, [i32, vp, vp, i32]),
+        "gmms_seg_autotune": (i32, [i32, i32, i32, i64, i32, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -369,3 +399,153 @@ def stage_times():
     _check(lib().gmms_stage_times(kinds.ctypes.data, ms.ctypes.data, el.ctypes.data, n, ctypes.byref(cnt)),
            "gmms_stage_times")
     return [(int(kinds[i]), float(ms[i]), int(el[i])) for i in range(n)]
+
+
+# ------------------------------------------------------- batched segments
+def _offsets_array(offsets):
+    import numpy as np
+
+    a = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
+    if a.ndim != 1 or a.size < 1:
+        raise GmmsError(ERROR_INVALID_VALUE, "offsets must be a 1-D sequence of n_seg + 1 entries")
+    return a
+
+
+class SegPlan:
+    """A planned batch of segments (gmms_seg_plan): segment s is
+    keys[offsets[s]:offsets[s+1]]. Planning is host work; sort() copies the
+    plan's tables into a device workspace and launches on the stream."""
+
+    def __init__(self, dtype: int, offsets, params: Optional[Params] = None):
+        self.dtype = dtype
+        self.offsets = _offsets_array(offsets)
+        h = ctypes.c_void_p()
+        st = lib().gmms_seg_plan_create(dtype, self.offsets.ctypes.data, self.offsets.size - 1, _pp(params),
+                                        ctypes.byref(h))
+        _check(st, "gmms_seg_plan_create")
+        self._h = h
+        self.info = SegInfo()
+        _check(lib().gmms_seg_plan_info(self._h, ctypes.byref(self.info)), "gmms_seg_plan_info")
+        self._keep = []
+
+    def segment(self, i: int) -> Tuple[int, int]:
+        """(K, merge rounds) chosen for segment i."""
+        k, r = ctypes.c_int(0), ctypes.c_int(0)
+        _check(lib().gmms_seg_plan_segment(self._h, i, ctypes.byref(k), ctypes.byref(r)), "gmms_seg_plan_segment")
+        return k.value, r.value
+
+    def sort(self, keys, keys_alt, values=None, values_alt=None, *, workspace: Optional[Workspace] = None,
+             stream=None) -> int:
+        """Sort every segment; returns 0 if the results are in keys/values,
+        1 if in keys_alt/values_alt. Asynchronous; keep the plan alive until
+        the stream has run it."""
+        _check_dev(keys, keys_alt, values, values_alt)
+        ws = workspace or _default_ws
+        wp, wb = ws.ensure(self.info.workspace_bytes, keys.device)
+        alt = ctypes.c_int(0)
+        st = lib().gmms_seg_sort(self._h, _ptr(keys), _ptr(keys_alt), _ptr(values), _ptr(values_alt), wp, wb,
+                                 _stream_ptr(stream), ctypes.byref(alt))
+        _check(st, "gmms_seg_sort")
+        # the plan's pinned tables feed an asynchronous copy: hold the buffers too
+        return alt.value
+
+    def graph(self, keys, keys_alt, values=None, values_alt=None) -> "SegGraph":
+        """Capture the batch's launches into a HIP graph bound to these buffers."""
+        return SegGraph(self, keys, keys_alt, values, values_alt)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().gmms_seg_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SegGraph:
+    """gmms_seg_graph: the batch's launches as one HIP graph; launch() re-sorts
+    whatever the bound buffers hold."""
+
+    def __init__(self, plan: SegPlan, keys, keys_alt, values=None, values_alt=None):
+        torch = _torch()
+        _check_dev(keys, keys_alt, values, values_alt)
+        self.plan = plan
+        self.ws = torch.empty(max(plan.info.workspace_bytes, 1), dtype=torch.uint8, device=keys.device)
+        self.buffers = (keys, keys_alt, values, values_alt)
+        h = ctypes.c_void_p()
+        st = lib().gmms_seg_graph_create(plan._h, _ptr(keys), _ptr(keys_alt), _ptr(values), _ptr(values_alt),
+                                         self.ws.data_ptr(), self.ws.numel(), ctypes.byref(h))
+        _check(st, "gmms_seg_graph_create")
+        self._h = h
+
+    def launch(self, stream=None) -> int:
+        alt = ctypes.c_int(0)
+        _check(lib().gmms_seg_graph_launch(self._h, _stream_ptr(stream), ctypes.byref(alt)), "gmms_seg_graph_launch")
+        return alt.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().gmms_seg_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def segmented_sort(keys, offsets, values=None, *, params: Optional[Params] = None, stream=None):
+    """Sort each segment keys[offsets[s]:offsets[s+1]] of a device tensor
+    (copy; synchronous). Returns the sorted keys, or (keys, values) for pairs."""
+    torch = _torch()
+    offs = _offsets_array(offsets)
+    k = keys.contiguous().clone()
+    ka = torch.empty_like(k)
+    v = va = None
+    dt = dtype_of(k)
+    if values is not None:
+        v = values.contiguous().clone()
+        va = torch.empty_like(v)
+        dt = PAIR_U64_U32
+    if offs[-1] > k.numel():
+        raise GmmsError(ERROR_INVALID_VALUE, "offsets exceed the tensor")
+    alt = ctypes.c_int(0)
+    st = lib().gmms_segmented_sort(dt, _ptr(k), _ptr(ka), _ptr(v), _ptr(va), offs.ctypes.data, offs.size - 1,
+                                   _pp(params), _stream_ptr(stream), ctypes.byref(alt))
+    _check(st, "gmms_segmented_sort")
+    if values is None:
+        return ka if alt.value else k
+    return (ka, va) if alt.value else (k, v)
+
+
+def seg_tuning_get(dtype: int):
+    """(K per size class, merger size per size class) of the batched planner."""
+    kk = (ctypes.c_int32 * SEG_CLASSES)()
+    mm = (ctypes.c_int64 * SEG_CLASSES)()
+    _check(lib().gmms_seg_tuning_get(dtype, ctypes.addressof(kk), ctypes.addressof(mm), SEG_CLASSES),
+           "gmms_seg_tuning_get")
+    return list(kk), list(mm)
+
+
+def seg_tuning_set(dtype: int, kway, merger_elems) -> None:
+    kk = (ctypes.c_int32 * SEG_CLASSES)(*list(kway)[:SEG_CLASSES])
+    mm = (ctypes.c_int64 * SEG_CLASSES)(*list(merger_elems)[:SEG_CLASSES])
+    _check(lib().gmms_seg_tuning_set(dtype, ctypes.addressof(kk), ctypes.addressof(mm), SEG_CLASSES),
+           "gmms_seg_tuning_set")
+
+
+def seg_autotune(dtype: int, min_class: int, max_class: int, batch_elems: int = 1 << 24, reps: int = 2,
+                 stream=None):
+    """Run the batched planner's autotuner and install its choices; returns
+    {class: (K, merger_elems, ms_per_batch)}."""
+    kk = (ctypes.c_int32 * SEG_CLASSES)()
+    mm = (ctypes.c_int64 * SEG_CLASSES)()
+    tt = (ctypes.c_float * SEG_CLASSES)()
+    st = lib().gmms_seg_autotune(dtype, min_class, max_class, batch_elems, reps, _stream_ptr(stream),
+                                 ctypes.addressof(kk), ctypes.addressof(mm), ctypes.addressof(tt))
+    _check(st, "gmms_seg_autotune")
+    return {c: (kk[c], mm[c], tt[c]) for c in range(min_class, max_class + 1)}

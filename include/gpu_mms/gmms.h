@@ -175,6 +175,88 @@ int gmms_find_partitions(int dtype, const void* keys, const void* vals, int64_t 
  * needs padding (ragged tiles are padded in registers). */
 int gmms_pad(int dtype, void* keys, void* vals, int64_t count, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Batched segmented sort. No counterpart in the reference, whose driver
+ * (src/multimergesort.hxx:56-141) sorts one array per call; this runs that
+ * same schedule for many arrays at once. Segment s is
+ * keys[offsets[s] .. offsets[s+1]) (offsets non-decreasing, offsets[0] >= 0,
+ * n_seg + 1 entries); each segment comes out exactly as gmms_sort would
+ * leave it on its own. Every segment uses the K of its size class (the
+ * tuning table below); the segments are aligned so that all of them finish
+ * in the same last round, and one launch sorts every 1024-element tile of
+ * every segment, then each round is one partition launch and one persistent
+ * merge launch whose wavefronts take (segment, task, merger) items from a
+ * device work queue, largest first. Launch count: 2 + at most 2 per round of
+ * the largest segment, independent of n_seg.
+ * ------------------------------------------------------------------------ */
+typedef struct gmms_seg_plan gmms_seg_plan;
+typedef struct gmms_seg_graph gmms_seg_graph;
+
+typedef struct gmms_seg_info {
+  int64_t n_seg;            /* segments (empty ones included)                */
+  int64_t elements;         /* sum of the segment lengths                    */
+  int64_t extent;           /* offsets[n_seg]: buffers hold at least this    */
+  int64_t tiles;            /* 1024-element tiles over all segments          */
+  int64_t merge_items;      /* mergers over all rounds                       */
+  int64_t partition_items;  /* partition searches over all rounds            */
+  size_t workspace_bytes;   /* device workspace gmms_seg_sort needs          */
+  int32_t rounds;           /* merge rounds (of the largest segment)         */
+  int32_t launches;         /* kernel launches per gmms_seg_sort             */
+} gmms_seg_info;
+
+/* Plans a batch from host-side offsets (host work only: no HIP call; the
+ * first gmms_seg_sort copies the plan's tables into pinned host memory that
+ * the plan keeps for its later sorts).
+ * p: kway = 0 takes K per size class from the tuning table, kway > 0 forces
+ * one K for all segments; p->mergers > 0 sets the resident merger count the
+ * rounds are split for. Only the wavefront merge kernel (pipeline 0) runs
+ * batches. */
+int gmms_seg_plan_create(int dtype, const int64_t* offsets, int64_t n_seg, const gmms_params* p,
+                         gmms_seg_plan** plan);
+int gmms_seg_plan_info(const gmms_seg_plan* plan, gmms_seg_info* info);
+/* K and number of merge rounds chosen for segment `seg`. */
+int gmms_seg_plan_segment(const gmms_seg_plan* plan, int64_t seg, int* kway, int* rounds);
+void gmms_seg_plan_destroy(gmms_seg_plan* plan);
+
+/* Sorts the planned batch on `stream`: copies the plan's tables into the
+ * workspace (workspace_bytes >= info.workspace_bytes) and launches. Buffers
+ * as for gmms_sort, extent elements each; *alt = 1 when the result is in
+ * keys_alt/vals_alt (the same for every segment). Asynchronous; the plan and
+ * the workspace must stay alive until the stream has finished the call.
+ * Elements before offsets[0] are neither read nor written. */
+int gmms_seg_sort(gmms_seg_plan* plan, void* keys, void* keys_alt, void* vals, void* vals_alt, void* workspace,
+                  size_t workspace_bytes, void* stream, int* alt);
+
+/* The same launches captured once into a HIP graph bound to these buffers
+ * and this workspace (the tables are copied at creation); every
+ * gmms_seg_graph_launch re-sorts whatever the buffers hold. */
+int gmms_seg_graph_create(gmms_seg_plan* plan, void* keys, void* keys_alt, void* vals, void* vals_alt,
+                          void* workspace, size_t workspace_bytes, gmms_seg_graph** graph);
+int gmms_seg_graph_launch(gmms_seg_graph* graph, void* stream, int* alt);
+void gmms_seg_graph_destroy(gmms_seg_graph* graph);
+
+/* One-shot: plan, allocate the workspace, sort, synchronize, free. */
+int gmms_segmented_sort(int dtype, void* keys, void* keys_alt, void* vals, void* vals_alt, const int64_t* offsets,
+                        int64_t n_seg, const gmms_params* p, void* stream, int* alt);
+
+/* Tuning table of the batched sort, per dtype and size class c = 0..40
+ * (segments with 2^(c-1) < n <= 2^c; class 0 holds n <= 1): the K of the
+ * segments of that class (0: the default, 16) and the merger size in
+ * elements the planner splits their tasks into (0: by the round's volume
+ * over the resident mergers). */
+#define GMMS_SEG_CLASSES 41
+int gmms_seg_tuning_get(int dtype, int32_t* kway, int64_t* merger_elems, int classes);
+int gmms_seg_tuning_set(int dtype, const int32_t* kway, const int64_t* merger_elems, int classes);
+/* Autotuner: for each class in [min_class, max_class], sorts batches of
+ * random segments of that size (about batch_elems elements per batch) with
+ * every candidate K (4..64 within the LDS window) and merger size, times
+ * them on `stream` with HIP events (best of `reps`) and installs the fastest
+ * into the tuning table. kway_out/merger_out/ms_out (may be NULL) receive
+ * the chosen entries and their time per batch, indexed by class.
+ * Allocates its own buffers; synchronous. */
+int gmms_seg_autotune(int dtype, int min_class, int max_class, int64_t batch_elems, int reps, void* stream,
+                      int32_t* kway_out, int64_t* merger_out, float* ms_out);
+
 #ifdef __cplusplus
 }
 #endif
