@@ -18,7 +18,17 @@ Reported:
   cpu_baseline  std::sort (1 thread, on a 2^25-key sample) and libstdc++
              parallel sort (all host threads, full input) of the same keys,
              timed on this host; the parallel figure is the reported one.
-The GPU output of input 0 is compared bit-exact with the CPU sort.
+  batched    the batched-segment workload: int32 segments with lengths
+             log-uniform in [2^10, 2^22] (mt19937_64 seed 0x6D6D7303), values
+             in [0, 2^31), cut at the first prefix reaching 2^28 keys (or all
+             8192 segments with --batched-literal). The autotuner picks K and
+             merger sizes per size class first; then the batch is sorted as
+             one HIP graph (and eagerly through gmms_seg_sort) and, for
+             comparison, with one plain sort call per segment. Timed with HIP
+             events on the launch stream; inputs are restored untimed.
+The GPU output of input 0 is compared bit-exact with the CPU sort; the
+batched output equals the per-segment output and every segment is compared
+with the CPU sort of that segment.
 """
 import argparse
 import json
@@ -46,7 +56,16 @@ def parse():
     ap.add_argument("--kernel", type=int, default=None,
                     help="merge kernel: 0 wavefront tile (default), 1 minBlockHeap, 2 workgroup tile")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-literal", action="store_true",
+                    help="all 8192 segments (about 4e9 keys) instead of the 2^28-key prefix")
+    ap.add_argument("--no-autotune", action="store_true")
     return ap.parse_args()
+
+
+def log(msg: str) -> None:
+    """Progress of the run on stderr (the JSON line alone goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def traffic_from_profiles(kernel: int, K: int, n: int):
@@ -61,6 +80,133 @@ def traffic_from_profiles(kernel: int, K: int, n: int):
     if d.get("pipeline") != kernel or d.get("K") != K or d.get("n") != n:
         return None
     return d.get("hbm_bytes_per_merge_launch")
+
+
+def batched(args, oracle, dev, stream, threads):
+    """Batched segmented sort vs one plain sort per segment (same keys)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+    import torch
+
+    import gmms
+
+    seed = 0x6D6D7303
+    lens = oracle.gen_seglens(seed, 8192, 10, 22)
+    if not args.batched_literal:
+        csum = np.cumsum(lens)
+        cut = int(np.searchsorted(csum, 1 << 28)) + 1  # first prefix whose sum reaches 2^28
+        lens = lens[:cut]
+    offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    total = int(offs[-1])
+    host = oracle.gen_i32(seed, total, 33)  # (int32)(u64 >> 33): [0, 2^31)
+    src = torch.from_numpy(host).to(dev)
+    k = torch.empty_like(src)
+    ka = torch.empty_like(src)
+
+    log(f"batched: {len(lens)} segments, {total} keys generated")
+    tuned = None
+    if not args.no_autotune:
+        res = gmms.seg_autotune(gmms.INT32, 10, 22, batch_elems=1 << 24, reps=2, stream=stream)
+        tuned = {str(c): {"K": K, "merger_elems": m, "ms_per_2^24_batch": round(ms, 4)} for c, (K, m, ms) in res.items()}
+        log("batched: autotune done")
+    plan = gmms.SegPlan(gmms.INT32, offs)
+    info = plan.info
+    ws = gmms.Workspace()
+    graph = plan.graph(k, ka)
+    reps = max(1, args.steps // 2)
+
+    def timed(fn):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        k.copy_(src)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        alt = fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1), alt
+
+    # warm-up of every path
+    timed(lambda: graph.launch(stream))
+    timed(lambda: plan.sort(k, ka, workspace=ws, stream=stream))
+    g_ms = min(timed(lambda: graph.launch(stream))[0] for _ in range(reps))
+    e_ms = min(timed(lambda: plan.sort(k, ka, workspace=ws, stream=stream))[0] for _ in range(reps))
+    k.copy_(src)
+    alt = graph.launch(stream)  # the batched result checked below
+    torch.cuda.synchronize()
+    batched_out = (ka if alt else k).clone()
+
+    # one plain sort call per segment, on the same buffers
+    per_ws = gmms.Workspace()
+    wmax = max(gmms.workspace_size(gmms.INT32, int(n)) for n in lens)
+    per_ws.ensure(max(wmax, 1), dev)
+    per_out = torch.empty_like(src)
+
+    # the loop calls the C entry point directly (no per-call Python wrapper work)
+    import ctypes
+
+    L = gmms.lib()
+    wp, wb = per_ws.tensor.data_ptr(), per_ws.tensor.numel()
+    sp = stream.cuda_stream
+    kp, kap = k.data_ptr(), ka.data_ptr()
+    seg_args = [(int(offs[s]) * 4, int(offs[s + 1] - offs[s])) for s in range(len(lens))]
+    alt_c = ctypes.c_int(0)
+
+    def per_array():
+        for s, (byte_off, n) in enumerate(seg_args):
+            st = L.gmms_sort(gmms.INT32, kp + byte_off, kap + byte_off, None, None, n, None, wp, wb, sp,
+                             ctypes.byref(alt_c))
+            if st:
+                raise gmms.GmmsError(st, "gmms_sort")
+            per_alt[s] = alt_c.value
+        return None
+
+    per_alt = [0] * len(lens)
+    timed(per_array)
+    p_ms = min(timed(per_array)[0] for _ in range(max(1, reps // 2)))
+    for s in range(len(lens)):
+        a, b = int(offs[s]), int(offs[s + 1])
+        per_out[a:b] = (ka if per_alt[s] else k)[a:b]
+    torch.cuda.synchronize()
+    same = bool(torch.equal(per_out, batched_out))
+
+    log(f"batched: per-array {p_ms:.3f} ms")
+    # every segment against the CPU sort of that segment
+    got = batched_out.cpu().numpy()
+    want = host.copy()
+
+    def cpu_sort(s):
+        a, b = int(offs[s]), int(offs[s + 1])
+        oracle.lib.oracle_sort_i32(want[a:b].ctypes.data, b - a)
+
+    with ThreadPoolExecutor(max_workers=max(1, min(16, threads))) as ex:
+        list(ex.map(cpu_sort, range(len(lens))))
+    exact = bool(np.array_equal(got, want))
+    rounds_per = [plan.segment(s)[1] for s in range(len(lens))]
+    per_launches = sum(1 + 2 * r for r in rounds_per)  # at most: base case + (partition + merge) per round
+    formula_bytes = float(sum(2.0 * int(n) * 4 * (1 + r) for n, r in zip(lens, rounds_per)))
+    graph.close()
+    return {
+        "workload": ("batched segments: %d int32 arrays, lengths log-uniform in [2^10, 2^22], "
+                     "values in [0, 2^31)%s" % (len(lens), "" if args.batched_literal else
+                                                 ", first prefix reaching 2^28 keys of the 8192-segment draw")),
+        "segments": len(lens),
+        "keys": total,
+        "rounds": info.rounds,
+        "launches_batched": info.launches,
+        "launches_per_array_max": per_launches,
+        "graph_ms": round(g_ms, 4),
+        "eager_ms": round(e_ms, 4),
+        "per_array_ms": round(p_ms, 4),
+        "batched_gkeys_s": round(total / (g_ms * 1e-3) / 1e9, 4),
+        "batched_eager_gkeys_s": round(total / (e_ms * 1e-3) / 1e9, 4),
+        "per_array_gkeys_s": round(total / (p_ms * 1e-3) / 1e9, 4),
+        "speedup_vs_per_array": round(p_ms / g_ms, 3),
+        "hbm_formula_GBps": round(formula_bytes / (g_ms * 1e-3) / 1e9, 1),
+        "bit_exact_vs_std_sort": exact,
+        "identical_to_per_array": same,
+        "autotune": tuned,
+    }
 
 
 def main():
@@ -115,6 +261,7 @@ def main():
     stages = gmms.stage_times()
     gmms.stage_timing(False)
     ms_per_step = (t1 - t0) * 1e3 / args.steps
+    log(f"plain sort: {ms_per_step:.3f} ms per step")
     gkeys = n / (ms_per_step * 1e-3) / 1e9
 
     # per-stage averages (kind -> list of ms), merges grouped by round position
@@ -159,6 +306,7 @@ def main():
     formula_bytes = 2.0 * n * 4 * (1 + rounds)
     sort_gbps = formula_bytes / (ms_per_step * 1e-3) / 1e9
 
+    log(f"bit-exact check done: {bit_exact}")
     cpu = None
     if not args.no_cpu_baseline:
         sample = min(n, 1 << 25)
@@ -173,6 +321,12 @@ def main():
             "single_thread": {"value": round(sample / t_single / 1e9, 4), "unit": "GKeys/s", "cores": 1,
                               "sample": f"std::sort of the first {sample} keys"},
         }
+
+    if cpu:
+        log("cpu baseline done")
+    batch = None
+    if not args.no_batched:
+        batch = batched(args, oracle, dev, stream, threads)
 
     out = {
         "metric": METRIC,
@@ -204,9 +358,10 @@ def main():
                           "frac_of_published_peak": round(sort_gbps / PUBLISHED_HBM_PEAK_GBPS, 4)},
         "stage_ms_avg": {k: round(sum(v) / len(v), 4) for k, v in per_kind.items()},
         "merge_round_ms": per_round,
+        "batched": batch,
     }
     print(json.dumps(out), flush=True)
-    if not bit_exact:
+    if not bit_exact or (batch is not None and not (batch["bit_exact_vs_std_sort"] and batch["identical_to_per_array"])):
         sys.exit(1)
 
 
