@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/gpu_mms/gmms.h"
+#include "gmms_basecase.hxx"
 #include "gmms_elem.hxx"
 #include "gmms_merge.hxx"
 #include "gmms_tmerge.hxx"
@@ -207,5 +208,13 @@ inline int dispatch(int dtype, F&& f) {
     default: return GMMS_ERROR_INVALID_VALUE;
   }
 }
+
+// Batched-sort internals used by the server (gmms_serve.hip): the plan's
+// host table block, where it goes in the workspace, and the launches alone
+// (the caller has copied the table block to ws + *ws_offset on the stream).
+const void* seg_plan_table(const gmms_seg_plan* plan, size_t* bytes, size_t* ws_offset);
+size_t seg_plan_workspace(const gmms_seg_plan* plan);
+int seg_plan_launch(const gmms_seg_plan* plan, void* keys, void* keys_alt, void* vals, void* vals_alt, void* ws,
+                    hipStream_t s, int* alt);
 
 }  // namespace gmms_host

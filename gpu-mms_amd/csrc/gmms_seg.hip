@@ -413,6 +413,27 @@ __global__ void k_fill_random(uint32_t* words, int64_t n_words, uint64_t seed) {
 
 }  // namespace
 
+namespace gmms_host {
+
+const void* seg_plan_table(const gmms_seg_plan* P, size_t* bytes, size_t* ws_offset) {
+  *bytes = P->host_bytes;
+  *ws_offset = P->off_table;
+  return P->table.data();
+}
+
+size_t seg_plan_workspace(const gmms_seg_plan* P) { return P->ws_bytes; }
+
+int seg_plan_launch(const gmms_seg_plan* P, void* keys, void* keys_alt, void* vals, void* vals_alt, void* ws,
+                    hipStream_t s, int* alt) {
+  if (alt) *alt = P->R & 1;
+  if (P->n_segs_dev == 0) return GMMS_SUCCESS;
+  return dispatch(P->dtype, [&](auto dt) {
+    return seg_enqueue<decltype(dt)::value>(P, keys, keys_alt, vals, vals_alt, (char*)ws, s, false);
+  });
+}
+
+}  // namespace gmms_host
+
 extern "C" {
 
 int gmms_seg_plan_create(int dtype, const int64_t* offsets, int64_t n_seg, const gmms_params* p,

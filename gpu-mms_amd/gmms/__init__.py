@@ -87,6 +87,37 @@ class SegInfo(ctypes.Structure):
 
 SEG_CLASSES = 41  # GMMS_SEG_CLASSES
 
+
+class ServerConfig(ctypes.Structure):
+    """gmms_server_config (gmms.h)."""
+
+    _fields_ = [
+        ("hbm_budget", ctypes.c_size_t),
+        ("max_batch_elems", ctypes.c_int64),
+        ("max_batch_requests", ctypes.c_int32),
+        ("max_inflight", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    """gmms_server_stats (gmms.h)."""
+
+    _fields_ = [
+        ("submitted", ctypes.c_int64),
+        ("completed", ctypes.c_int64),
+        ("rejected", ctypes.c_int64),
+        ("failed", ctypes.c_int64),
+        ("batches", ctypes.c_int64),
+        ("max_batch_size", ctypes.c_int64),
+        ("max_inflight_seen", ctypes.c_int64),
+        ("keys_sorted", ctypes.c_int64),
+        ("budget", ctypes.c_size_t),
+        ("arena_bytes", ctypes.c_size_t),
+        ("reserved_bytes", ctypes.c_size_t),
+        ("peak_reserved_bytes", ctypes.c_size_t),
+    ]
+
 _lib = None
 
 
@@ -140,6 +171,16 @@ def lib() -> ctypes.CDLL:
         "gmms_seg_tuning_get": (i32, [i32, vp, vp, i32]),
         "gmms_seg_tuning_set": (i32, [i32, vp, vp, i32]),
         "gmms_seg_autotune": (i32, [i32, i32, i32, i64, i32, vp, vp, vp, vp]),
+        "gmms_server_default_config": (i32, [ctypes.POINTER(ServerConfig)]),
+        "gmms_server_request_bytes": (i32, [i32, i64, ctypes.POINTER(sz)]),
+        "gmms_server_create": (i32, [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]),
+        "gmms_server_submit": (i32, [vp, i32, vp, vp, i64, vp, vp, ctypes.POINTER(ctypes.c_uint64)]),
+        "gmms_server_wait": (i32, [vp, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(i32), ctypes.POINTER(i32),
+                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]),
+        "gmms_server_drain": (i32, [vp]),
+        "gmms_server_stats_get": (i32, [vp, ctypes.POINTER(ServerStats)]),
+        "gmms_server_destroy": (None, [vp]),
+        "gmms_server_batch_policy": (i32, [vp, vp, i64, sz, i64, i32, vp, ctypes.POINTER(i64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -549,3 +590,125 @@ def seg_autotune(dtype: int, min_class: int, max_class: int, batch_elems: int = 
                                  ctypes.addressof(kk), ctypes.addressof(mm), ctypes.addressof(tt))
     _check(st, "gmms_seg_autotune")
     return {c: (kk[c], mm[c], tt[c]) for c in range(min_class, max_class + 1)}
+
+
+# ------------------------------------------------------------- sort server
+def server_request_bytes(dtype: int, n: int) -> int:
+    b = ctypes.c_size_t(0)
+    _check(lib().gmms_server_request_bytes(dtype, n, ctypes.byref(b)), "gmms_server_request_bytes")
+    return b.value
+
+
+def server_batch_policy(dtypes, sizes, free_bytes: int, max_batch_elems: int = 1 << 28,
+                        max_batch_requests: int = 4096):
+    """The server's batching policy alone (host logic): indices of the next
+    batch among the pending requests (arrival order)."""
+    import numpy as np
+
+    d = np.ascontiguousarray(np.asarray(dtypes, np.int32))
+    n = np.ascontiguousarray(np.asarray(sizes, np.int64))
+    out = np.zeros(max(1, d.size), np.int64)
+    cnt = ctypes.c_int64(0)
+    st = lib().gmms_server_batch_policy(d.ctypes.data, n.ctypes.data, d.size, free_bytes, max_batch_elems,
+                                        max_batch_requests, out.ctypes.data, ctypes.byref(cnt))
+    _check(st, "gmms_server_batch_policy")
+    return [int(i) for i in out[:cnt.value]]
+
+
+class Ticket:
+    """A submitted request: its id and the host buffers it reads and fills."""
+
+    def __init__(self, tid, keys, values, out_keys, out_values):
+        self.id = tid
+        self.inputs = (keys, values)
+        self.out_keys = out_keys
+        self.out_values = out_values
+        self.status = None
+        self.latency_ms = None
+        self.batch = None
+
+
+class Server:
+    """Sort server (gmms_server): requests are host tensors (pinned memory
+    lets copies overlap), results come back in pinned host tensors."""
+
+    def __init__(self, hbm_budget: int = 0, max_batch_elems: int = 0, max_batch_requests: int = 0,
+                 max_inflight: int = 0):
+        cfg = ServerConfig()
+        _check(lib().gmms_server_default_config(ctypes.byref(cfg)), "gmms_server_default_config")
+        if hbm_budget:
+            cfg.hbm_budget = hbm_budget
+        if max_batch_elems:
+            cfg.max_batch_elems = max_batch_elems
+        if max_batch_requests:
+            cfg.max_batch_requests = max_batch_requests
+        if max_inflight:
+            cfg.max_inflight = max_inflight
+        h = ctypes.c_void_p()
+        _check(lib().gmms_server_create(ctypes.byref(cfg), ctypes.byref(h)), "gmms_server_create")
+        self._h = h
+        self.config = cfg
+
+    def submit(self, keys, values=None, *, dtype: Optional[int] = None) -> Ticket:
+        """Queue a sort of host tensor `keys` (and `values` for pairs)."""
+        torch = _torch()
+        if keys.is_cuda or (values is not None and values.is_cuda):
+            raise GmmsError(ERROR_INVALID_VALUE, "server requests are host tensors")
+        keys = keys.contiguous()
+        dt = dtype if dtype is not None else (PAIR_U64_U32 if values is not None else dtype_of(keys))
+        out_k = torch.empty(keys.shape, dtype=keys.dtype, pin_memory=True)
+        out_v = None
+        if values is not None:
+            values = values.contiguous()
+            out_v = torch.empty(values.shape, dtype=values.dtype, pin_memory=True)
+        tid = ctypes.c_uint64(0)
+        st = lib().gmms_server_submit(self._h, dt, keys.data_ptr(), _ptr(values), keys.numel(), out_k.data_ptr(),
+                                      _ptr(out_v), ctypes.byref(tid))
+        _check(st, "gmms_server_submit")
+        return Ticket(tid.value, keys, values, out_k, out_v)
+
+    def wait(self, t: Ticket, timeout: Optional[float] = None) -> bool:
+        """Wait for a request; True once it is done (then t.status,
+        t.latency_ms and t.batch are set)."""
+        if t.status is not None:
+            return True
+        done, status = ctypes.c_int(0), ctypes.c_int(0)
+        lat, batch = ctypes.c_double(0), ctypes.c_int64(-1)
+        st = lib().gmms_server_wait(self._h, t.id, -1.0 if timeout is None else float(timeout), ctypes.byref(done),
+                                    ctypes.byref(status), ctypes.byref(lat), ctypes.byref(batch))
+        _check(st, "gmms_server_wait")
+        if not done.value:
+            return False
+        t.status, t.latency_ms, t.batch = status.value, lat.value, batch.value
+        return True
+
+    def result(self, t: Ticket):
+        """Wait, check the status, return the sorted keys (or keys, values)."""
+        self.wait(t)
+        _check(t.status, "server request")
+        return t.out_keys if t.out_values is None else (t.out_keys, t.out_values)
+
+    def drain(self) -> None:
+        _check(lib().gmms_server_drain(self._h), "gmms_server_drain")
+
+    def stats(self) -> ServerStats:
+        s = ServerStats()
+        _check(lib().gmms_server_stats_get(self._h, ctypes.byref(s)), "gmms_server_stats_get")
+        return s
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().gmms_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -257,6 +257,78 @@ int gmms_seg_tuning_set(int dtype, const int32_t* kway, const int64_t* merger_el
 int gmms_seg_autotune(int dtype, int min_class, int max_class, int64_t batch_elems, int reps, void* stream,
                       int32_t* kway_out, int64_t* merger_out, float* ms_out);
 
+/* ------------------------------------------------------------------------
+ * Sort server. No counterpart in the reference (its harness, src/testMM.cu,
+ * sorts one array per process). Requests (host buffers of any supported
+ * dtype) are admitted against an HBM budget: the server allocates one device
+ * arena of exactly hbm_budget bytes at creation and carves every batch out
+ * of it, so device memory in use never exceeds the budget and no request
+ * allocates. A scheduler thread batches pending requests continuously: as
+ * soon as an in-flight slot and arena space are free it takes the oldest
+ * pending request plus the following pending requests of the same dtype
+ * (up to max_batch_requests / max_batch_elems, as long as the batch fits in
+ * the arena's largest free block), copies them in, sorts them as segments
+ * of one batched segmented sort, copies the results out and frees the
+ * space. Up to max_inflight batches run at once, each on its own stream.
+ * A request that can never fit (needs more than the budget) is rejected
+ * with GMMS_ERROR_BUDGET at submission. Host buffers should be pinned
+ * (hipHostMalloc) for copies to overlap; pageable ones work.
+ * ------------------------------------------------------------------------ */
+typedef struct gmms_server gmms_server;
+
+typedef struct gmms_server_config {
+  size_t hbm_budget;           /* device bytes the server may use (default 64 GB)  */
+  int64_t max_batch_elems;     /* elements per batch (default 2^28)                */
+  int32_t max_batch_requests;  /* requests per batch (default 4096)                */
+  int32_t max_inflight;        /* batches on the device at once (default 3)        */
+  int32_t reserved[4];
+} gmms_server_config;
+
+typedef struct gmms_server_stats {
+  int64_t submitted, completed, rejected, failed;
+  int64_t batches;              /* batches launched                                 */
+  int64_t max_batch_size;       /* most requests in one batch                       */
+  int64_t max_inflight_seen;    /* most batches on the device at once               */
+  int64_t keys_sorted;          /* elements of completed requests                   */
+  size_t budget;                /* hbm_budget                                       */
+  size_t arena_bytes;           /* device bytes allocated by the server (= budget)  */
+  size_t reserved_bytes;        /* arena bytes held by in-flight batches now        */
+  size_t peak_reserved_bytes;   /* most arena bytes ever held at once               */
+} gmms_server_stats;
+
+int gmms_server_default_config(gmms_server_config* cfg);
+/* Device bytes one request of n elements holds while in flight (both key
+ * buffers, both value buffers for pairs; the batch's tables come on top). */
+int gmms_server_request_bytes(int dtype, int64_t n, size_t* bytes);
+int gmms_server_create(const gmms_server_config* cfg, gmms_server** server);
+/* Queue a request: sort n elements of keys (and vals for pairs) into
+ * out_keys (out_vals); input and output buffers must stay valid until the
+ * request completes. *ticket identifies it. Returns at once. */
+int gmms_server_submit(gmms_server* server, int dtype, const void* keys, const void* vals, int64_t n,
+                       void* out_keys, void* out_vals, uint64_t* ticket);
+/* Wait up to timeout_s seconds (< 0: forever) for a request; *done = 1 once
+ * it has finished with status *status; latency_ms (may be NULL) receives
+ * submission-to-completion time and batch_id (may be NULL) the batch that
+ * sorted it. A finished request is forgotten after the first wait that
+ * reports it done. */
+int gmms_server_wait(gmms_server* server, uint64_t ticket, double timeout_s, int* done, int* status,
+                     double* latency_ms, int64_t* batch_id);
+/* Wait until every submitted request has finished. */
+int gmms_server_drain(gmms_server* server);
+int gmms_server_stats_get(gmms_server* server, gmms_server_stats* stats);
+void gmms_server_destroy(gmms_server* server);
+
+/* The batching policy alone (host logic, no device): given the pending
+ * requests in arrival order (dtype[i], n[i]), the bytes of the largest free
+ * arena block and the limits, writes the indices of the next batch to
+ * chosen[] (*n_chosen of them; 0 if the oldest request does not fit now):
+ * the oldest request, then later ones of its dtype, in order, while the
+ * batch's data bytes (gmms_server_request_bytes) stay within free_bytes and
+ * the limits. Requests of other dtypes are skipped, not blocked. */
+int gmms_server_batch_policy(const int32_t* dtype, const int64_t* n, int64_t n_pending, size_t free_bytes,
+                             int64_t max_batch_elems, int32_t max_batch_requests, int64_t* chosen,
+                             int64_t* n_chosen);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,120 @@
+"""GPU tests of the sort server: every result is bit-identical alone and in a
+mixed batch (and equal to the oracle's sort), the HBM budget bounds the
+device memory in use, batches overlap, over-budget requests are refused."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _pinned(a):
+    import torch
+
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    return t.pin_memory()
+
+
+def _bits(x):
+    return x.view({4: np.uint32, 8: np.uint64}[x.dtype.itemsize])
+
+
+def _requests(oracle):
+    reqs = []
+    for i, n in enumerate([1, 1000, 4096, 70_001, 300_000, 1 << 20]):
+        reqs.append(("i32", oracle.gen_i32(900 + i, n, 32), None))
+    for i, n in enumerate([777, 65_536, 250_001]):
+        x = oracle.gen_f32_normal(950 + i, n)
+        x.view(np.uint32)[::61] = 0x80000000  # -0
+        x.view(np.uint32)[1::61] = 0x7FC00000  # NaN
+        reqs.append(("f32", x, None))
+    for i, n in enumerate([5000, 131_073]):
+        k, v = oracle.gen_pairs(970 + i, n, 3, 1.1, 1 << 20)
+        reqs.append(("pair", k, v))
+    return reqs
+
+
+def _submit(srv, kind, k, v):
+    import gmms
+
+    if kind == "pair":
+        return srv.submit(_pinned(k.view(np.int64)), _pinned(v.view(np.int32)), dtype=gmms.PAIR_U64_U32)
+    return srv.submit(_pinned(k))
+
+
+def _host(t):
+    if isinstance(t, tuple):
+        return t[0].numpy().view(np.uint64), t[1].numpy().view(np.uint32)
+    return t.numpy()
+
+
+def test_results_same_alone_and_in_mix(gmms, oracle, dev):
+    print("server: alone vs mix", flush=True)
+    reqs = _requests(oracle)
+    with gmms.Server(hbm_budget=1 << 30) as srv:
+        alone = []
+        for kind, k, v in reqs:
+            t = _submit(srv, kind, k, v)
+            alone.append(_host(srv.result(t)))
+        # all at once: same-dtype requests share batches, batches overlap
+        ts = [_submit(srv, kind, k, v) for kind, k, v in reqs]
+        mixed = [_host(srv.result(t)) for t in ts]
+        st = srv.stats()
+        assert st.completed == 2 * len(reqs) and st.failed == 0
+        assert st.max_batch_size >= 2
+        assert len({t.batch for t in ts}) < len(ts)
+    for (kind, k, v), a, m in zip(reqs, alone, mixed):
+        if kind == "pair":
+            wk, wv = oracle.sort_pairs(k, v)
+            assert np.array_equal(a[0], wk) and np.array_equal(a[1], wv)
+            assert np.array_equal(m[0], a[0]) and np.array_equal(m[1], a[1])
+        else:
+            w = oracle.sort(k)
+            assert np.array_equal(_bits(a), _bits(w))
+            assert np.array_equal(_bits(m), _bits(a))
+
+
+def test_budget_bounds_memory_and_batches_overlap(gmms, oracle, dev):
+    print("server: 48 MB budget, 40 requests", flush=True)
+    budget = 48 << 20
+    n = 1 << 20  # 8 MiB of key buffers per request: at most 5 in flight
+    with gmms.Server(hbm_budget=budget, max_batch_requests=2, max_inflight=3) as srv:
+        srcs = [oracle.gen_i32(2000 + i, n, 32) for i in range(40)]
+        ts = [srv.submit(_pinned(a)) for a in srcs]
+        outs = [srv.result(t).numpy() for t in ts]
+        st = srv.stats()
+        assert st.arena_bytes == budget and st.budget == budget
+        assert 0 < st.peak_reserved_bytes <= budget
+        assert st.reserved_bytes == 0
+        assert st.batches >= 20 and st.max_batch_size <= 2
+        assert st.max_inflight_seen >= 2
+        assert st.keys_sorted == 40 * n
+    for a, o in zip(srcs[:5], outs[:5]):
+        assert np.array_equal(o, oracle.sort(a))
+    for o in outs:
+        assert (np.diff(o.astype(np.int64)) >= 0).all()
+
+
+def test_over_budget_request_is_refused(gmms, dev):
+    import torch
+
+    print("server: refuse over-budget", flush=True)
+    with gmms.Server(hbm_budget=16 << 20) as srv:
+        with pytest.raises(gmms.GmmsError) as e:
+            srv.submit(torch.zeros(4 << 20, dtype=torch.int32))  # 32 MiB of key buffers
+        assert e.value.status == gmms.ERROR_BUDGET
+        t = srv.submit(torch.arange(1000, 0, -1, dtype=torch.int32).pin_memory())
+        assert np.array_equal(srv.result(t).numpy(), np.arange(1, 1001, dtype=np.int32))
+        t0 = srv.submit(torch.zeros(0, dtype=torch.int32))
+        assert srv.wait(t0) and t0.status == 0
+        st = srv.stats()
+        assert st.rejected == 1 and st.completed == 2
+
+
+def test_wait_timeout_and_drain(gmms, oracle, dev):
+    print("server: wait with timeout, drain", flush=True)
+    with gmms.Server(hbm_budget=256 << 20) as srv:
+        ts = [srv.submit(_pinned(oracle.gen_i32(3000 + i, 1 << 18, 32))) for i in range(8)]
+        srv.drain()
+        for t in ts:
+            assert srv.wait(t, timeout=0.0) and t.status == 0 and t.latency_ms > 0
+        assert srv.stats().completed == 8
