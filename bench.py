@@ -26,6 +26,12 @@ Reported:
              one HIP graph (and eagerly through gmms_seg_sort) and, for
              comparison, with one plain sort call per segment. Timed with HIP
              events on the launch stream; inputs are restored untimed.
+  serving    the sort server under Poisson arrivals (offered rate
+             --serve-rate requests/s, --serve-requests of them), sizes
+             log-uniform in [2^10, 2^26] keys, dtypes 60% int32 / 25% float32
+             (total order) / 15% (uint64 key, uint32 value) pairs, uniform
+             random values, 64 GB HBM budget, pinned host buffers; p50/p99
+             latency (submission to results back in host memory) and keys/s.
 The GPU output of input 0 is compared bit-exact with the CPU sort; the
 batched output equals the per-segment output and every segment is compared
 with the CPU sort of that segment.
@@ -60,6 +66,9 @@ def parse():
     ap.add_argument("--batched-literal", action="store_true",
                     help="all 8192 segments (about 4e9 keys) instead of the 2^28-key prefix")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=160)
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="offered requests per second")
     return ap.parse_args()
 
 
@@ -161,6 +170,7 @@ def batched(args, oracle, dev, stream, threads):
             per_alt[s] = alt_c.value
         return None
 
+    log(f"batched: graph {g_ms:.3f} ms, eager {e_ms:.3f} ms")
     per_alt = [0] * len(lens)
     timed(per_array)
     p_ms = min(timed(per_array)[0] for _ in range(max(1, reps // 2)))
@@ -206,6 +216,123 @@ def batched(args, oracle, dev, stream, threads):
         "bit_exact_vs_std_sort": exact,
         "identical_to_per_array": same,
         "autotune": tuned,
+    }
+
+
+def serving(args, oracle, threads):
+    """Poisson arrivals into the sort server; every result checked."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+    import torch
+
+    import gmms
+
+    rng = np.random.default_rng(0x6D6D7305)
+    nreq = args.serve_requests
+    sizes = np.floor(2.0 ** rng.uniform(10, 26, nreq)).astype(np.int64)
+    kinds = rng.choice(3, nreq, p=[0.60, 0.25, 0.15])  # int32, float32, pairs
+    gaps = rng.exponential(1.0 / args.serve_rate, nreq)
+    reqs = []
+    for i in range(nreq):
+        n = int(sizes[i])
+        if kinds[i] == 0:
+            k = torch.from_numpy(rng.integers(-2**31, 2**31, n, dtype=np.int64).astype(np.int32)).pin_memory()
+            reqs.append((k, None))
+        elif kinds[i] == 1:
+            bits = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+            k = torch.from_numpy(bits.view(np.float32)).pin_memory()  # every float32 bit pattern
+            reqs.append((k, None))
+        else:
+            k = torch.from_numpy(rng.integers(0, 2**63, n, dtype=np.int64)).pin_memory()
+            v = torch.from_numpy(np.arange(n, dtype=np.int32)).pin_memory()
+            reqs.append((k, v))
+    log(f"serving: {nreq} requests generated ({int(sizes.sum())} keys)")
+    srv = gmms.Server()  # 64 GB budget
+
+    # saturation: every request submitted at once (closed batch), for the
+    # server's peak rate on this mix
+    ts0 = time.perf_counter()
+    sat = [srv.submit(k) if v is None else srv.submit(k, v, dtype=gmms.PAIR_U64_U32) for k, v in reqs]
+    srv.drain()
+    sat_wall = time.perf_counter() - ts0
+    for t in sat:
+        srv.wait(t)
+    sat_lat = np.array([t.latency_ms for t in sat])
+    sat_ok = all(t.status == 0 for t in sat)
+    sat_batches = srv.stats().batches
+    del sat
+    log(f"serving: saturated run {sat_wall:.3f} s")
+
+    tickets = []
+    t0 = time.perf_counter()
+    due = t0
+    for i, (k, v) in enumerate(reqs):
+        due += gaps[i]
+        while True:
+            now = time.perf_counter()
+            if now >= due:
+                break
+            time.sleep(min(due - now, 0.001))
+        if v is None:
+            tickets.append(srv.submit(k))
+        else:
+            tickets.append(srv.submit(k, v, dtype=gmms.PAIR_U64_U32))
+    t_sub = time.perf_counter()
+    srv.drain()
+    t_end = time.perf_counter()
+    for t in tickets:
+        srv.wait(t)
+    st = srv.stats()
+    srv.close()
+    lat = np.array([t.latency_ms for t in tickets])
+    status_ok = all(t.status == 0 for t in tickets)
+    log(f"serving: drained after {t_end - t0:.3f} s; checking results")
+
+    def check(i):
+        k, v = reqs[i]
+        t = tickets[i]
+        if v is None:
+            a = k.numpy()
+            want = oracle.sort(a)
+            got = t.out_keys.numpy()
+            bits = {4: np.uint32, 8: np.uint64}[a.dtype.itemsize]
+            return bool(np.array_equal(got.view(bits), want.view(bits)))
+        wk, wv = oracle.sort_pairs(k.numpy().view(np.uint64), v.numpy().view(np.uint32))
+        return bool(np.array_equal(t.out_keys.numpy().view(np.uint64), wk) and
+                    np.array_equal(t.out_values.numpy().view(np.uint32), wv))
+
+    with ThreadPoolExecutor(max_workers=max(1, min(16, threads))) as ex:
+        exact = all(ex.map(check, range(nreq)))
+    total = int(sizes.sum())
+    wall = t_end - t0
+    return {
+        "workload": ("sort server: Poisson arrivals at %.0f requests/s offered, %d requests, sizes log-uniform in "
+                     "[2^10, 2^26], dtypes 60%% int32 / 25%% float32 / 15%% (u64,u32) pairs, 64 GB budget"
+                     % (args.serve_rate, nreq)),
+        "requests": nreq,
+        "keys": total,
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "mean_ms": round(float(lat.mean()), 3),
+        "max_ms": round(float(lat.max()), 3),
+        "keys_per_s": round(total / wall, 1),
+        "gkeys_s": round(total / wall / 1e9, 4),
+        "wall_s": round(wall, 4),
+        "submit_span_s": round(t_sub - t0, 4),
+        "batches": st.batches - sat_batches,
+        "mean_batch_requests": round((st.completed - nreq) / max(1, st.batches - sat_batches), 2),
+        "max_batch_requests": st.max_batch_size,
+        "max_inflight_batches": st.max_inflight_seen,
+        "budget_bytes": st.budget,
+        "peak_reserved_bytes": st.peak_reserved_bytes,
+        "all_status_ok": status_ok and sat_ok,
+        "bit_exact_vs_std_sort": exact,
+        "saturated": {"gkeys_s": round(total / sat_wall / 1e9, 4), "wall_s": round(sat_wall, 4),
+                      "p50_ms": round(float(np.percentile(sat_lat, 50)), 3),
+                      "p99_ms": round(float(np.percentile(sat_lat, 99)), 3),
+                      "batches": sat_batches,
+                      "note": "all requests submitted at once; results of this run are not re-checked"},
     }
 
 
@@ -328,6 +455,10 @@ def main():
     if not args.no_batched:
         batch = batched(args, oracle, dev, stream, threads)
 
+    serve = None
+    if not args.no_serving:
+        serve = serving(args, oracle, threads)
+
     out = {
         "metric": METRIC,
         "value": round(gkeys, 4),
@@ -359,9 +490,12 @@ def main():
         "stage_ms_avg": {k: round(sum(v) / len(v), 4) for k, v in per_kind.items()},
         "merge_round_ms": per_round,
         "batched": batch,
+        "serving": serve,
     }
     print(json.dumps(out), flush=True)
     if not bit_exact or (batch is not None and not (batch["bit_exact_vs_std_sort"] and batch["identical_to_per_array"])):
+        sys.exit(1)
+    if serve is not None and not (serve["bit_exact_vs_std_sort"] and serve["all_status_ok"]):
         sys.exit(1)
 
 
