@@ -12,7 +12,8 @@
  *             minBlockHeap, 2 tile merge with one merger per workgroup
  *   RANGE     range of the harness's random keys, rand() % RANGE
  *   BLOCKS    mergers per round when a round has few tasks (reference: 482;
- *             here 0 = two per compute unit, 512 on MI355X)
+ *             here 0 = auto: the resident merger count of the merge kernel,
+ *             12 per compute unit for the default one, 3072 on MI355X)
  *   M         base-case tile (1024 elements, one wavefront)
  *   W         wavefront width on CDNA4
  *   B         elements per heap node (one per lane)
