@@ -34,6 +34,15 @@ constexpr int wm_win(int W) { return W == 1 ? 1024 : 512; }
 
 constexpr int TM_F = 64;  // tile-merge window per list
 
+constexpr size_t LDS_LIMIT = 160 * 1024;  // gfx950 LDS per workgroup
+
+// resident wavefront mergers per CU for W-word elements
+inline int wm_resident_per_cu(int W) {
+  const size_t per_wg = WM_WAVES * wmerge_lds_words_per_wave(W, wm_win(W)) * sizeof(uint32_t);
+  const int wgs = (int)std::max<size_t>(1, LDS_LIMIT / per_wg);
+  return std::min(wgs * WM_WAVES, 32);
+}
+
 inline int ilog2(int x) {
   int r = 0;
   while ((1 << r) < x) ++r;
@@ -63,8 +72,6 @@ inline int elem_words(int dtype) {
 }
 inline bool valid_dtype(int dtype) { return elem_words(dtype) > 0; }
 inline bool has_vals(int dtype) { return dtype == GMMS_PAIR_U64_U32; }
-
-constexpr size_t LDS_LIMIT = 160 * 1024;  // gfx950 LDS per workgroup
 
 // K = 0 means: the largest power of two <= 64 whose heap + rings fit in LDS
 inline int resolve(int dtype, const gmms_params* p, Resolved* r, bool need_device) {
@@ -97,8 +104,9 @@ inline int resolve(int dtype, const gmms_params* p, Resolved* r, bool need_devic
   r->K = K;
   r->PL = ilog2(K);
   r->D = D;
-  // resident mergers per CU: heap 2 (LDS), workgroup tile 3, wavefront tile 12
-  const int per_cu = kern == GMMS_MERGE_HEAP ? 2 : (kern == GMMS_MERGE_TILE ? 3 : 12);
+  // resident mergers per CU: heap 2 (LDS), workgroup tile 3, wavefront tile
+  // as many workgroups of WM_WAVES mergers as fit the LDS
+  const int per_cu = kern == GMMS_MERGE_HEAP ? 2 : (kern == GMMS_MERGE_TILE ? 3 : wm_resident_per_cu(W));
   r->mergers = q.mergers ? q.mergers : per_cu * (need_device ? cu_count() : 256);
   r->slack = q.slack;
   return GMMS_SUCCESS;

@@ -132,6 +132,12 @@ __device__ __forceinline__ typename DType<DT>::E ld_int(const RoundArgs& a, int6
   return gload<DT, false>(a.in_keys, a.in_vals, i);
 }
 
+__device__ __forceinline__ int wave_sum32(int x) {
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) x += __shfl_xor(x, d, WAVE);
+  return x;
+}
+
 __device__ __forceinline__ int64_t wave_sum64(int64_t x) {
 #pragma unroll
   for (int d = 1; d < WAVE; d <<= 1) x += __shfl_xor(x, d, WAVE);

@@ -79,13 +79,6 @@ int max_seg_k(int W) { return std::min(MAX_K, wm_win(W) / 16); }
 
 constexpr int SEG_DEFAULT_K = 16;
 
-// resident wavefront mergers per CU for W-word elements
-int wm_resident_per_cu(int W) {
-  const size_t per_wg = WM_WAVES * wmerge_lds_words_per_wave(W, wm_win(W)) * sizeof(uint32_t);
-  const int wgs = (int)std::max<size_t>(1, LDS_LIMIT / per_wg);
-  return wgs * WM_WAVES;
-}
-
 struct SegRound {
   int64_t items_off, n_items;  // merger items (in MergerInfo units from the table start)
   int64_t part_off, n_part;    // partition items

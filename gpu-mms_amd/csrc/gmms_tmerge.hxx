@@ -40,12 +40,6 @@ __host__ __device__ inline size_t tmerge_lds_words(int K, int W, int F) {
   return 3 * cap * W + 2 * (2 * (size_t)K + 4) + 4 * (size_t)K + (K + 1) + 8;
 }
 
-__device__ __forceinline__ int wave_sum32(int x) {
-#pragma unroll
-  for (int d = 1; d < WAVE; d <<= 1) x += __shfl_xor(x, d, WAVE);
-  return x;
-}
-
 template <class E>
 __device__ __forceinline__ E pl_get(const uint32_t* base, size_t cap, int idx) {
   uint32_t w[E::W];

@@ -1,16 +1,26 @@
 // Wavefront tile K-way merge: every 64-lane wavefront is an independent
-// merger with its own LDS windows and merge buffers, so a merge step never
-// waits on a workgroup barrier and 12+ mergers share a CU to hide LDS and
-// HBM latency. Same contract as k_tmerge (elements between two consistent
-// cuts from k_partition, written at rank sum(lo)); same step:
+// merger with its own LDS windows, so a merge step never waits on a
+// workgroup barrier and 16 mergers share a CU to hide LDS and HBM latency.
+// A merger merges the elements between two consistent cuts from k_partition
+// and writes them at rank sum(lo). One step:
 //   refill windows -> frontier y (smallest last element over full windows) ->
-//   gallop if a single list holds everything <= y, else gather the K taken
-//   prefixes and merge them with log2 K stable merge-path passes -> store.
-// Differences: lane a owns active list a (empty lists are compacted away,
-// keeping list order for stability), and the window per list is
-// WIN / pow2ceil(active lists), so a merger with few active lists (the last
-// round, presorted input) gets long windows instead of short steps.
+//   gallop if a single list holds everything <= y, else take every list's
+//   prefix <= y into a register tile and merge the runs with bitonic merge
+//   levels -> store.
+// Lane a owns active list a (empty lists are compacted away), and the window
+// per list is F = WIN / pow2ceil(active lists), so a merger with few active
+// lists (the last round, presorted input) gets long windows instead of short
+// steps.
+// The register tile has WIN slots, NREG = WIN/64 per lane (slot s in lane
+// s/NREG, register s%NREG); run r fills slots [r*F, r*F + F): its taken
+// prefix, then the maximum element. Each run is sorted, so the tile is Kp
+// sorted blocks of F and the merge levels of the base-case network from
+// block size F up to WIN sort it; the first S slots are the S taken elements
+// in order. Filler slots hold the all-ones element, which equals the
+// largest possible real element bit for bit, so a real maximum and a filler
+// are interchangeable and the first S slots are exact.
 #pragma once
+#include "gmms_basecase.hxx"
 #include "gmms_elem.hxx"
 #include "gmms_merge.hxx"
 
@@ -18,9 +28,10 @@ namespace gmms {
 
 constexpr int WM_WAVES = 4;  // mergers per workgroup
 
-// LDS words per wavefront merger with a window budget of WIN elements.
+// LDS words per wavefront merger with a window budget of WIN elements: the
+// windows, and the bank-padded transpose of the register tile for the store.
 __host__ __device__ inline size_t wmerge_lds_words_per_wave(int W, int WIN) {
-  return 3 * (size_t)WIN * W + 4 * WAVE + 8;
+  return (size_t)WIN * W + (size_t)WAVE * (WIN / WAVE + 1) * W;
 }
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -58,81 +69,17 @@ __device__ __forceinline__ void lput(uint32_t* base, int cap, int idx, const E& 
   for (int q = 0; q < E::W; ++q) base[q * cap + idx] = word(e, q);
 }
 
-// One stable pairwise merge pass: groups of g runs (run r at s_off[r]) are
-// merged in halves; lane l writes outputs [l*Eo, (l+1)*Eo). FROM_WIN: the
-// runs are list prefixes in the circular windows (run r = window r from
-// s_head[r]) instead of being packed in src. WM_CAP: the word-plane stride.
-// Reads one past a run's end stay inside this merger's LDS (index <= WM_CAP)
-// and are never selected.
-template <class E, bool FROM_WIN, int WM_CAP>
-__device__ __forceinline__ void merge_pass(const uint32_t* src, uint32_t* dst, int g, int Kp, int S, int Eo, int lane,
-                                           const int* s_off, const int* s_head, int F) {
-  int o = lane * Eo;
-  const int oend = min(S, o + Eo);
-  while (o < oend) {
-    int l = 0, h = Kp / g - 1;
-    while (l < h) {
-      const int mid = (l + h + 1) >> 1;
-      const bool c = s_off[mid * g] <= o;
-      l = c ? mid : l;
-      h = c ? h : mid - 1;
-    }
-    const int gs = s_off[l * g], gm = s_off[l * g + g / 2], ge = s_off[l * g + g];
-    const int n1 = gm - gs, n2 = ge - gm;
-    const int d = o - gs;
-    // element addresses of the two halves
-    int b1, b2, h1 = 0, h2 = 0;
-    if (FROM_WIN) {
-      b1 = (l * g) * F;  // g == 2: runs 2l and 2l+1
-      b2 = b1 + F;
-      h1 = s_head[l * g];
-      h2 = s_head[l * g + 1];
-    } else {
-      b1 = gs;
-      b2 = gm;
-    }
-    const int msk = F - 1;
-    auto adr1 = [&](int i) { return FROM_WIN ? b1 + ((h1 + i) & msk) : b1 + i; };
-    auto adr2 = [&](int i) { return FROM_WIN ? b2 + ((h2 + i) & msk) : b2 + i; };
-    int plo = d - n2 > 0 ? d - n2 : 0;
-    int phi = d < n1 ? d : n1;
-    while (plo < phi) {
-      const int mid = (plo + phi) >> 1;
-      // A1[mid] is among the first d outputs iff A1[mid] <= A2[d-1-mid] (left run first on ties)
-      const bool c = !lt(lget<E>(src, WM_CAP, adr2(d - 1 - mid)), lget<E>(src, WM_CAP, adr1(mid)));
-      plo = c ? mid + 1 : plo;
-      phi = c ? phi : mid;
-    }
-    int i = plo, jj = d - plo;
-    const int cntm = min(oend, ge) - o;
-    E va = lget<E>(src, WM_CAP, adr1(i));
-    E vb = lget<E>(src, WM_CAP, adr2(jj));
-    for (int e = 0; e < cntm; ++e) {
-      const bool ta = (jj >= n2) | ((i < n1) & !lt(vb, va));
-      lput<E>(dst, WM_CAP, o + e, ta ? va : vb);
-      i += ta ? 1 : 0;
-      jj += ta ? 0 : 1;
-      const E nv = lget<E>(src, WM_CAP, ta ? adr1(i) : adr2(jj));
-      va = ta ? nv : va;
-      vb = ta ? vb : nv;
-    }
-    o += cntm;
-  }
-}
-
 // One merger, executed by one wavefront using the LDS region `my`.
 template <int DT, bool CONV_OUT, int WIN>
 __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo& mi, int lane, uint32_t* my) {
   using E = typename DType<DT>::E;
   constexpr int W = E::W;
   constexpr int CHUNKS = WIN / WAVE;
-  uint32_t* win = my;                 // [list a][F] circular windows, W planes of WIN
-  uint32_t* bufA = win + WIN * W;
-  uint32_t* bufB = bufA + WIN * W;
-  int* st = (int*)(bufB + WIN * W);   // per active list: head, taken, off (+1)
-  int* s_head = st;
-  int* s_take = st + WAVE;
-  int* s_off = st + 2 * WAVE;         // WAVE + 1 entries
+  constexpr int NREG = WIN / WAVE;        // register-tile slots per lane
+  constexpr int XS = NREG + 1;            // padded transpose stride
+  constexpr int XW = WAVE * XS;           // transpose words per plane
+  uint32_t* win = my;                     // [list a][F] circular windows, W planes of WIN
+  uint32_t* xp = win + WIN * W;           // transpose of the register tile
 
   const int K = mi.K;
   const int j = mi.j;
@@ -178,8 +125,6 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
     if (__ballot(want > 0) == 0) break;
 
     // ---- refill: chunk u covers window slots u*64 .. u*64+63
-    if (lane < Ka) s_head[lane] = head;
-    wave_lds_sync();
     {
       // all loads in flight first; remember which slots got one
       E v[CHUNKS];
@@ -321,50 +266,51 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
       continue;
     }
 
-    // ---- offsets of the taken prefixes
-    int incl = sk;
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) {
-      const int o = __shfl_up(incl, d, WAVE);
-      incl += lane >= d ? o : 0;
-    }
-    const int S = __shfl(incl, WAVE - 1, WAVE);
-    s_take[lane] = sk;
-    s_off[lane] = incl - sk;
-    if (lane == WAVE - 1) s_off[WAVE] = S;
-    wave_lds_sync();
-
-    // ---- stable pairwise merge passes over Kp runs at s_off[]; the first
-    // pass reads the runs straight out of the circular windows
-    const int Eo = (S + WAVE - 1) / WAVE;
+    // ---- elements taken this step
+    const int S = wave_sum32(sk);
     if (Kp == 1) {
       // one run: it is window 0 (list 0's, held by lane 0), in order from its head
       const int h0 = __builtin_amdgcn_readlane(head, 0);
       for (int o = lane; o < S; o += WAVE)
         gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + o, lget<E>(win, WIN, (h0 + o) & (F - 1)));
     } else {
-      // gather the taken prefixes into bufA (run r at s_off[r])
+      // register tile: lane l holds slots l*NREG .. l*NREG+NREG-1, all in run r
+      const int r = (lane * NREG) / F;
+      const int i0 = (lane * NREG) & (F - 1);
+      const int tk = __shfl(sk, r, WAVE);
+      const int hd = __shfl(head, r, WAVE);
+      E v[NREG];
 #pragma unroll
-      for (int u = 0; u < CHUNKS; ++u) {
-        const int slot = u * WAVE + lane;
-        const int l = slot / F;
-        const int i = slot & (F - 1);  // logical index within list l's prefix
-        if (l < Ka) {
-          const int tl = s_take[l];
-          if (i < tl) lput<E>(bufA, WIN, s_off[l] + i, lget<E>(win, WIN, l * F + ((s_head[l] + i) & (F - 1))));
-        }
+      for (int q = 0; q < NREG; ++q) {
+        const E x = lget<E>(win, WIN, r * F + ((hd + i0 + q) & (F - 1)));
+        v[q] = i0 + q < tk ? x : Limits<E>::hi();
+      }
+      // merge the Kp sorted blocks of F slots: levels from block F up to WIN
+      const int b0 = __builtin_ctz((unsigned)(F / NREG));
+      switch (b0) {
+        case 0: merge_level<0>(v, lane); [[fallthrough]];
+        case 1: merge_level<1>(v, lane); [[fallthrough]];
+        case 2: merge_level<2>(v, lane); [[fallthrough]];
+        case 3: merge_level<3>(v, lane); [[fallthrough]];
+        case 4: merge_level<4>(v, lane); [[fallthrough]];
+        default: merge_level<5>(v, lane);
+      }
+      // transpose through LDS so the stores are coalesced
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+#pragma unroll
+        for (int q = 0; q < NREG; ++q) xp[w * XW + lane * XS + q] = word(v[q], w);
       }
       wave_lds_sync();
-      uint32_t* src = bufA;
-      uint32_t* dst = bufB;
-      for (int g = 2; g <= Kp; g <<= 1) {
-        merge_pass<E, false, WIN>(src, dst, g, Kp, S, Eo, lane, s_off, s_head, F);
-        wave_lds_sync();
-        uint32_t* t = src;
-        src = dst;
-        dst = t;
+#pragma unroll
+      for (int u = 0; u < NREG; ++u) {
+        const int o = u * WAVE + lane;
+        const int slot = (o / NREG) * XS + (o % NREG);
+        uint32_t wd[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) wd[w] = xp[w * XW + slot];
+        if (o < S) gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + o, from_words<E>(wd));
       }
-      for (int o = lane; o < S; o += WAVE) gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + o, lget<E>(src, WIN, o));
     }
 
     // ---- advance
@@ -378,7 +324,7 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
 
 // Plain sort: one merger per wavefront over the uniform task grid.
 template <int DT, bool CONV_OUT, int WIN>
-__global__ __launch_bounds__(WM_WAVES* WAVE) void k_wmerge(RoundArgs a) {
+__global__ __launch_bounds__(WM_WAVES* WAVE, 4) void k_wmerge(RoundArgs a) {
   using E = typename DType<DT>::E;
   extern __shared__ uint32_t lds[];
   const int lane = threadIdx.x & (WAVE - 1);
@@ -394,7 +340,7 @@ __global__ __launch_bounds__(WM_WAVES* WAVE) void k_wmerge(RoundArgs a) {
 // exhausted (every wavefront reaches that exit). The counter starts at zero
 // (cleared by the launch before).
 template <int DT, bool CONV_OUT, int WIN>
-__global__ __launch_bounds__(WM_WAVES* WAVE) void k_wmerge_persistent(RoundArgs a) {
+__global__ __launch_bounds__(WM_WAVES* WAVE, 4) void k_wmerge_persistent(RoundArgs a) {
   using E = typename DType<DT>::E;
   extern __shared__ uint32_t lds[];
   const int lane = threadIdx.x & (WAVE - 1);
