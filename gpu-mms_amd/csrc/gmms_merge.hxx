@@ -18,11 +18,14 @@
 // precedes every element after it, so the merge of the lists from the cut on
 // is exactly the task's sorted sequence from rank R = sum(c_k). The search
 // keeps two consistent cuts lo (R_lo <= r) and hi (R_hi > r) and narrows them
-// with pivots drawn from the lists' windows -- interpolation steps alternated
-// with weighted-median steps (each of which removes >= 1/4 of the window) --
-// until r - R_lo <= slack. The merger then discards the first r - R_lo
-// outputs. Each pivot needs one windowed binary search per list, so the cost
-// is a few binary searches per list per merger; no list's pivot is ever
+// with one pivot per step: the weighted median (weights: the lists' window
+// widths) of per-list candidates placed by interpolation for rank r, which
+// halves r - R_lo or R_hi - r; after two such steps in a row that only lower
+// R_hi, the candidates are the window midpoints instead (a median step,
+// which removes >= 1/4 of the window) -- until r - R_lo <= slack. The heap
+// merger then discards the first r - R_lo outputs (the tile mergers start at
+// the cut itself). Each pivot needs one windowed search per list, so the
+// cost is a few searches per list per merger; no list's pivot is ever
 // written by an idle merger (the reference's race, warp-partition.hxx:522-531).
 //
 // Merge. A minBlockHeap of 2K-1 nodes x 64 elements (one element per lane)
@@ -163,13 +166,16 @@ __device__ __forceinline__ void partition_one(const RoundArgs& a, const MergerIn
   const int64_t lbase = mi.tbase + (int64_t)lane * mi.L;
 
   int64_t lo = 0, hi = nk, Rlo = 0, Rhi = T;
+  int hi_moves = 0;  // consecutive interpolation steps that only lowered hi
   for (int it = 0; it < 256 && r - Rlo > slack; ++it) {
     const int64_t Wt = Rhi - Rlo;  // == sum(hi - lo)
     const int64_t w = hi - lo;
     const bool act = w > 0;
-    // interpolation on even steps (target r), weighted median on odd steps
-    const bool interp = (it & 1) == 0;
-    const int64_t tw = interp ? (r - Rlo) : (Wt >> 1);  // in [0, Wt)
+    // candidates: each list's position interpolated for rank r, or its
+    // midpoint (a median step, after two interpolation steps in a row that
+    // only lowered hi); the pivot is their weighted median either way
+    const bool interp = hi_moves < 2;
+    const int64_t tw = Wt >> 1;  // in [0, Wt)
     int64_t pos = lo;
     if (act) {
       int64_t off = interp ? (int64_t)((double)(r - Rlo) / (double)Wt * (double)w) : (w >> 1);
@@ -203,21 +209,42 @@ __device__ __forceinline__ void partition_one(const RoundArgs& a, const MergerIn
     for (int q = 0; q < E::W; ++q) yw[q] = (uint32_t)__shfl((int)word(x, q), kp, WAVE);
     const E y = from_words<E>(yw);
     const int64_t ppos = __shfl(pos, kp, WAVE);
-    // count of list elements ordered before y, inside [lo, hi]
-    int64_t c;
-    if (lane == kp) {
-      c = ppos;
-    } else {
-      int64_t l = lo, h = hi;  // answer in [l, h]
-      const bool upper = lane < kp;  // equal keys of lower lists come first
-      while (l < h) {
-        const int64_t mid = l + ((h - l) >> 1);
-        const E v = ld_int<DT>(a, lbase + mid);
-        const bool before = upper ? !lt(y, v) : lt(v, y);
-        if (before) l = mid + 1; else h = mid;
+    // count of list elements ordered before y, inside [lo, hi]: every list
+    // is searched by G = 64/K lanes at once, an A-ary search (A = max(G, 2),
+    // probes g = 0..A-2 at l + (g+1)*n/A; powers of two, so shifts only)
+    const int G = WAVE / K;
+    const int ab = G >= 2 ? __builtin_ctz((unsigned)G) : 1;  // log2(A)
+    const int A = 1 << ab;
+    const int gl = lane / G;  // this lane's list
+    const int gs = lane % G;  // its probe (gs <= A-2 probes)
+    int64_t l = __shfl(lo, gl, WAVE), h = __shfl(hi, gl, WAVE);
+    if (gl == kp) l = h = ppos;     // the pivot's own list: exactly its position
+    const bool upper = gl < kp;     // equal keys of lower lists come first
+    const int64_t gbase = mi.tbase + (int64_t)gl * mi.L;
+    const unsigned long long gmask = G == WAVE ? ~0ull : ((1ull << G) - 1ull);
+    for (;;) {
+      const bool open = l < h;
+      if (__ballot(open) == 0) break;
+      const int64_t n = h - l;
+      const bool probe = open && gs <= A - 2;
+      bool before = true;
+      if (probe) {
+        const E v = ld_int<DT>(a, gbase + l + (((int64_t)(gs + 1) * n) >> ab));
+        before = upper ? !lt(y, v) : lt(v, y);
       }
-      c = l;
+      const unsigned long long nb = (__ballot(probe && !before) >> (gl * G)) & gmask;
+      if (open) {
+        if (nb == 0) {
+          l = l + (((int64_t)(A - 1) * n) >> ab) + 1;  // past the last probe
+        } else {
+          const int g0 = __ffsll((long long)nb) - 1;  // first probe not before y
+          const int64_t hp = l + (((int64_t)(g0 + 1) * n) >> ab);
+          l = g0 == 0 ? l : l + (((int64_t)g0 * n) >> ab) + 1;
+          h = hp;
+        }
+      }
     }
+    int64_t c = __shfl(l, (lane % K) * G, WAVE);
     if (!inl) c = 0;
     const int64_t R = wave_sum64(c);
     if (R <= r) {
@@ -225,9 +252,11 @@ __device__ __forceinline__ void partition_one(const RoundArgs& a, const MergerIn
       Rlo = R;
       if (R < r && lane == kp) lo = c + 1;  // the pivot itself is taken too
       if (R < r) Rlo = R + 1;
+      hi_moves = 0;
     } else {
       hi = c;
       Rhi = R;
+      hi_moves = interp ? hi_moves + 1 : 0;
     }
   }
   const int64_t b = mi.cut_row + j;

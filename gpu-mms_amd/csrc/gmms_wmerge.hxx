@@ -302,10 +302,12 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
         for (int q = 0; q < NREG; ++q) xp[w * XW + lane * XS + q] = word(v[q], w);
       }
       wave_lds_sync();
+      // slot of output o = u*64 + lane: lane (o / NREG), register (o % NREG)
+      const int xb = (lane / NREG) * XS + (lane % NREG);
 #pragma unroll
       for (int u = 0; u < NREG; ++u) {
         const int o = u * WAVE + lane;
-        const int slot = (o / NREG) * XS + (o % NREG);
+        const int slot = xb + u * (WAVE / NREG) * XS;
         uint32_t wd[W];
 #pragma unroll
         for (int w = 0; w < W; ++w) wd[w] = xp[w * XW + slot];
