@@ -69,6 +69,23 @@ __device__ __forceinline__ void lput(uint32_t* base, int cap, int idx, const E& 
   for (int q = 0; q < E::W; ++q) base[q * cap + idx] = word(e, q);
 }
 
+// LDS DMA of element gi into word planes dst[q * plane] (dst wave-uniform;
+// lane l's words land at dst[q * plane + l]).
+template <int DT>
+__device__ __forceinline__ void dma_elem(const RoundArgs& a, int64_t gi, uint32_t* dst, int plane) {
+  using E = typename DType<DT>::E;
+  typedef __attribute__((address_space(3))) void* lds_ptr;
+  if constexpr (E::W == 1) {
+    __builtin_amdgcn_global_load_lds((const void*)((const uint32_t*)a.in_keys + gi), (lds_ptr)dst, 4, 0, 0);
+  } else {
+    const uint32_t* kw = (const uint32_t*)a.in_keys + 2 * gi;
+    __builtin_amdgcn_global_load_lds((const void*)kw, (lds_ptr)dst, 4, 0, 0);
+    __builtin_amdgcn_global_load_lds((const void*)(kw + 1), (lds_ptr)(dst + plane), 4, 0, 0);
+    if constexpr (E::W == 3)
+      __builtin_amdgcn_global_load_lds((const void*)(a.in_vals + gi), (lds_ptr)(dst + 2 * plane), 4, 0, 0);
+  }
+}
+
 // One merger, executed by one wavefront using the LDS region `my`.
 template <int DT, bool CONV_OUT, int WIN>
 __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo& mi, int lane, uint32_t* my) {
@@ -117,60 +134,55 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
   int Kp = 1;
   while (Kp < Ka) Kp <<= 1;
   const int F = WIN / Kp;  // window per active list (power of two >= WIN/64)
-  int head = 0, cnt = 0;
-
-  for (;;) {
-    const int64_t rem = endp - cur;
-    const int want = live ? (int)(rem < F ? rem : F) : 0;
-    if (__ballot(want > 0) == 0) break;
-
-    // ---- refill: chunk u covers window slots u*64 .. u*64+63
-    {
-      // all loads in flight first; remember which slots got one
-      E v[CHUNKS];
-      uint32_t vmask = 0;
-      // Loads are unconditional (a slot with nothing to fetch reads element 0,
-      // always in bounds) so that no branch join forces a wait between them.
-      if (F >= WAVE) {
-        // a chunk lies inside one list: its state is a wave-uniform readlane
-#pragma unroll
-        for (int u = 0; u < CHUNKS; ++u) {
-          const int slot = u * WAVE + lane;
-          const int lr = (u * WAVE) / F;
-          const int l = lr < WAVE ? lr : WAVE - 1;
-          const int sj = slot & (F - 1);
-          const int hl = __builtin_amdgcn_readlane(head, l);
-          const int cl = __builtin_amdgcn_readlane(cnt, l);
-          const int wl = __builtin_amdgcn_readlane(want, l);
-          const int64_t cu = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(cur >> 32), l) << 32) |
-                                       (uint32_t)__builtin_amdgcn_readlane((int)cur, l));
-          const int i = (sj - hl) & (F - 1);
-          const bool ok = lr < Ka && i >= cl && i < wl;
-          v[u] = gload<DT, false>(a.in_keys, a.in_vals, ok ? cu + i : 0);
-          vmask |= ok ? (1u << u) : 0u;
-        }
-      } else {
-#pragma unroll
-        for (int u = 0; u < CHUNKS; ++u) {
-          const int slot = u * WAVE + lane;
-          const int l = slot / F;  // active list of this slot (< 64: F >= 16)
-          const int sj = slot & (F - 1);
-          const int hl = __shfl(head, l, WAVE);
-          const int cl = __shfl(cnt, l, WAVE);
-          const int wl = __shfl(want, l, WAVE);
-          const int64_t cu = __shfl(cur, l, WAVE);
-          const int i = (sj - hl) & (F - 1);  // logical index of this slot
-          const bool ok = l < Ka && i >= cl && i < wl;
-          v[u] = gload<DT, false>(a.in_keys, a.in_vals, ok ? cu + i : 0);
-          vmask |= ok ? (1u << u) : 0u;
-        }
-      }
+  // Window refill by LDS DMA: the logical range [c, w) of every active list
+  // (physical slots (h + i) & (F-1) of its window) is loaded from global
+  // memory straight into LDS; chunk u covers window slots u*64 .. u*64+63.
+  // Completion is awaited with vmcnt(0) later, so the loads of the next
+  // step overlap this step's merge network.
+  auto refill = [&](int64_t cu_, int h_, int c_, int w_) {
+    if (F >= WAVE) {
+      // a chunk lies inside one list: its state is a wave-uniform readlane
 #pragma unroll
       for (int u = 0; u < CHUNKS; ++u) {
-        if ((vmask >> u) & 1u) lput<E>(win, WIN, u * WAVE + lane, v[u]);
+        const int lr = (u * WAVE) / F;
+        if (lr < Ka) {
+          const int hl = __builtin_amdgcn_readlane(h_, lr);
+          const int cl = __builtin_amdgcn_readlane(c_, lr);
+          const int wl = __builtin_amdgcn_readlane(w_, lr);
+          const int64_t cu = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(cu_ >> 32), lr) << 32) |
+                                       (uint32_t)__builtin_amdgcn_readlane((int)cu_, lr));
+          const int i = ((u * WAVE + lane) - hl) & (F - 1);
+          if (i >= cl && i < wl) dma_elem<DT>(a, cu + i, win + u * WAVE, WIN);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < CHUNKS; ++u) {
+        const int slot = u * WAVE + lane;
+        const int l = slot / F;  // active list of this slot (< 64: F >= 16)
+        const int hl = __shfl(h_, l, WAVE);
+        const int cl = __shfl(c_, l, WAVE);
+        const int wl = __shfl(w_, l, WAVE);
+        const int64_t cu = __shfl(cu_, l, WAVE);
+        const int i = (slot - hl) & (F - 1);  // logical index of this slot
+        if (l < Ka && i >= cl && i < wl) dma_elem<DT>(a, cu + i, win + u * WAVE, WIN);
       }
     }
+  };
+  auto wait_dma = [&]() {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): every DMA (and store) done
     wave_lds_sync();
+  };
+
+  // window state: list a's window holds its next `want` elements, logical
+  // [0, want) at physical (head + i) & (F-1); cur = its first element
+  int head = 0;
+  int want = live ? (int)(endp - cur < F ? endp - cur : F) : 0;
+  refill(cur, head, 0, want);
+  wait_dma();
+
+  for (;;) {
+    if (__ballot(want > 0) == 0) break;
 
     // ---- frontier y = min over full windows of the last element, tag (elem, list)
     const bool full = live && want == F;
@@ -253,26 +265,48 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
     }
 
     if (gallop) {
+      // list ky's run goes straight from input to output, 4 elements per lane in flight
       const int64_t src = __shfl(cur, ky, WAVE);
-      for (int64_t i = lane; i < glen; i += WAVE)
+      int64_t i = lane;
+      for (; i + 3 * WAVE < glen; i += 4 * WAVE) {
+        E g[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) g[t] = gload<DT, false>(a.in_keys, a.in_vals, src + i + t * WAVE);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + i + t * WAVE, g[t]);
+      }
+      for (; i < glen; i += WAVE)
         gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + i, gload<DT, false>(a.in_keys, a.in_vals, src + i));
+      // list ky restarts with an empty window; the others keep theirs
+      int c2 = want;
       if (lane == ky) {
         cur += glen;
         head = 0;
-        cnt = 0;
+        c2 = 0;
       }
       obase += glen;
-      wave_lds_sync();
+      const int w2 = live ? (int)(endp - cur < F ? endp - cur : F) : 0;
+      refill(cur, head, c2, w2);
+      want = w2;
+      wait_dma();
       continue;
     }
 
-    // ---- elements taken this step
+    // ---- elements taken this step, and the state after it
     const int S = wave_sum32(sk);
+    const int64_t cur2 = cur + sk;
+    const int head2 = (head + sk) & (F - 1);
+    const int cnt2 = want - sk;
+    const int want2 = live ? (int)(endp - cur2 < F ? endp - cur2 : F) : 0;
+    // outputs o = u*64 + lane (o < S), in order
+    E ov[NREG];
     if (Kp == 1) {
       // one run: it is window 0 (list 0's, held by lane 0), in order from its head
       const int h0 = __builtin_amdgcn_readlane(head, 0);
-      for (int o = lane; o < S; o += WAVE)
-        gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + o, lget<E>(win, WIN, (h0 + o) & (F - 1)));
+#pragma unroll
+      for (int u = 0; u < NREG; ++u) ov[u] = lget<E>(win, WIN, (h0 + u * WAVE + lane) & (F - 1));
+      __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): window reads done before the DMA reuses the slots
+      refill(cur2, head2, cnt2, want2);
     } else {
       // register tile: lane l holds slots l*NREG .. l*NREG+NREG-1, all in run r
       const int r = (lane * NREG) / F;
@@ -285,6 +319,8 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
         const E x = lget<E>(win, WIN, r * F + ((hd + i0 + q) & (F - 1)));
         v[q] = i0 + q < tk ? x : Limits<E>::hi();
       }
+      __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): window reads done before the DMA reuses the slots
+      refill(cur2, head2, cnt2, want2);
       // merge the Kp sorted blocks of F slots: levels from block F up to WIN
       const int b0 = __builtin_ctz((unsigned)(F / NREG));
       switch (b0) {
@@ -306,21 +342,25 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
       const int xb = (lane / NREG) * XS + (lane % NREG);
 #pragma unroll
       for (int u = 0; u < NREG; ++u) {
-        const int o = u * WAVE + lane;
         const int slot = xb + u * (WAVE / NREG) * XS;
         uint32_t wd[W];
 #pragma unroll
         for (int w = 0; w < W; ++w) wd[w] = xp[w * XW + slot];
-        if (o < S) gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + o, from_words<E>(wd));
+        ov[u] = from_words<E>(wd);
       }
+    }
+    wait_dma();
+#pragma unroll
+    for (int u = 0; u < NREG; ++u) {
+      const int o = u * WAVE + lane;
+      if (o < S) gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + o, ov[u]);
     }
 
     // ---- advance
     obase += S;
-    cur += sk;
-    head = (head + sk) & (F - 1);
-    cnt = want - sk;
-    wave_lds_sync();
+    cur = cur2;
+    head = head2;
+    want = want2;
   }
 }
 
