@@ -60,21 +60,23 @@ template <int B, class E, int N>
 __device__ __forceinline__ void flip_lanes(E (&v)[N], int lane) {
   constexpr int M = (2 << B) - 1;
   const bool low = ((lane >> B) & 1) == 0;
+  const uint32_t hsel = low ? 0u : 0xFFFFFFFFu;
   E p[N];
 #pragma unroll
   for (int r = 0; r < N; ++r) p[r] = xlane<M>(v[N - 1 - r]);
 #pragma unroll
-  for (int r = 0; r < N; ++r) v[r] = low ? emin(v[r], p[r]) : emax(v[r], p[r]);
+  for (int r = 0; r < N; ++r) v[r] = lane_ce(v[r], p[r], low, hsel);
 }
 
 // half-cleaner across lanes at lane distance D
 template <int D, class E, int N>
 __device__ __forceinline__ void clean_lanes(E (&v)[N], int lane) {
   const bool low = (lane & D) == 0;
+  const uint32_t hsel = low ? 0u : 0xFFFFFFFFu;
 #pragma unroll
   for (int r = 0; r < N; ++r) {
     E p = xlane<D>(v[r]);
-    v[r] = low ? emin(v[r], p) : emax(v[r], p);
+    v[r] = lane_ce(v[r], p, low, hsel);
   }
 }
 

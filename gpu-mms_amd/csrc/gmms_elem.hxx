@@ -26,9 +26,11 @@ enum : int {
 // Cross-lane value of lane (lane ^ M) for one 32-bit word. Masks below 64
 // use DPP (quad_perm, row mirrors, row_ror:8) or ds_swizzle inside 32 lanes;
 // 32 and 63 cross the half-wave and go through ds_bpermute.
+// every lane of these patterns reads a valid source lane, so no old value
+// needs to be carried (bound_ctrl set, no copy into the destination first)
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp_mov(uint32_t x) {
-  return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, CTRL, 0xF, 0xF, false);
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, true);
 }
 template <int PAT>
 __device__ __forceinline__ uint32_t swizzle(uint32_t x) {
@@ -87,6 +89,26 @@ template <class E> __device__ __forceinline__ void cswap(E& a, E& b) {
 }
 
 template <class E> __device__ __forceinline__ E sel(bool c, const E& a, const E& b) { return c ? a : b; }
+
+// Cross-lane compare-exchange, one side: the lower lane of a pair keeps
+// min(x, p), the upper keeps max(x, p). hsel is 0 on lower lanes and all
+// ones on upper lanes: for 32-bit elements med3(x, p, hsel) is exactly that
+// (one v_med3_u32 instead of min, max and a select).
+__device__ __forceinline__ uint32_t med3_u32(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t r;
+  asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+template <class E>
+__device__ __forceinline__ E lane_ce(const E& x, const E& p, bool low, uint32_t hsel) {
+  (void)hsel;
+  return low ? emin(x, p) : emax(x, p);
+}
+template <>
+__device__ __forceinline__ E32 lane_ce(const E32& x, const E32& p, bool low, uint32_t hsel) {
+  (void)low;
+  return {med3_u32(x.k, p.k, hsel)};
+}
 
 template <class E> struct Limits;
 template <> struct Limits<E32> {
