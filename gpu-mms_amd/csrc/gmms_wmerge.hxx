@@ -69,20 +69,61 @@ __device__ __forceinline__ void lput(uint32_t* base, int cap, int idx, const E& 
   for (int q = 0; q < E::W; ++q) base[q * cap + idx] = word(e, q);
 }
 
-// LDS DMA of element gi into word planes dst[q * plane] (dst wave-uniform;
-// lane l's words land at dst[q * plane + l]).
+// Wave-uniform minimum of (x, k) over lanes 0..Kp-1 in (element, lane) order;
+// lanes without a candidate pass (max element, WAVE). k = WAVE: none.
+template <class E>
+__device__ __forceinline__ void frontier_min(E x, int k, int Kp, E& y, int& ky) {
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    if (d < Kp) {
+      const E ox = wxor(x, d);
+      const int ok = __shfl_xor(k, d, WAVE);
+      const bool take = ok < WAVE && (k == WAVE || lt(ox, x) || (!lt(x, ox) && ok < k));
+      x = take ? ox : x;
+      k = take ? ok : k;
+    }
+  }
+  uint32_t w[E::W];
+#pragma unroll
+  for (int q = 0; q < E::W; ++q) w[q] = (uint32_t)__builtin_amdgcn_readfirstlane((int)word(x, q));
+  y = from_words<E>(w);
+  ky = __builtin_amdgcn_readfirstlane(k);
+}
+// 32-bit elements: one 64-bit key (element << 7 | lane), neutral all ones
+template <>
+__device__ __forceinline__ void frontier_min<E32>(E32 x, int k, int Kp, E32& y, int& ky) {
+  uint64_t key = k == WAVE ? ~0ull : (((uint64_t)x.k << 7) | (uint64_t)k);
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    if (d < Kp) {
+      const uint64_t o = ((uint64_t)(uint32_t)__shfl_xor((int)(key >> 32), d, WAVE) << 32) |
+                         (uint32_t)__shfl_xor((int)key, d, WAVE);
+      key = o < key ? o : key;
+    }
+  }
+  const uint64_t kk = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(key >> 32)) << 32) |
+                      (uint32_t)__builtin_amdgcn_readfirstlane((int)key);
+  ky = kk == ~0ull ? WAVE : (int)(kk & 127u);
+  y.k = (uint32_t)(kk >> 7);
+}
+
+// LDS DMA of element base + off into word planes dst[q * plane] (base and
+// dst wave-uniform, off a per-lane 32-bit offset so the loads use the
+// scalar-base + vector-offset form; lane l's words land at dst[q * plane + l]).
 template <int DT>
-__device__ __forceinline__ void dma_elem(const RoundArgs& a, int64_t gi, uint32_t* dst, int plane) {
+__device__ __forceinline__ void dma_elem(const RoundArgs& a, int64_t base, uint32_t off, uint32_t* dst, int plane) {
   using E = typename DType<DT>::E;
   typedef __attribute__((address_space(3))) void* lds_ptr;
   if constexpr (E::W == 1) {
-    __builtin_amdgcn_global_load_lds((const void*)((const uint32_t*)a.in_keys + gi), (lds_ptr)dst, 4, 0, 0);
+    const uint32_t* kb = (const uint32_t*)a.in_keys + base;
+    __builtin_amdgcn_global_load_lds((const void*)(kb + off), (lds_ptr)dst, 4, 0, 0);
   } else {
-    const uint32_t* kw = (const uint32_t*)a.in_keys + 2 * gi;
+    const uint64_t* kb = (const uint64_t*)a.in_keys + base;
+    const uint32_t* kw = (const uint32_t*)(kb + off);
     __builtin_amdgcn_global_load_lds((const void*)kw, (lds_ptr)dst, 4, 0, 0);
     __builtin_amdgcn_global_load_lds((const void*)(kw + 1), (lds_ptr)(dst + plane), 4, 0, 0);
     if constexpr (E::W == 3)
-      __builtin_amdgcn_global_load_lds((const void*)(a.in_vals + gi), (lds_ptr)(dst + 2 * plane), 4, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(a.in_vals + base + off), (lds_ptr)(dst + 2 * plane), 4, 0, 0);
   }
 }
 
@@ -142,17 +183,20 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
   auto refill = [&](int64_t cu_, int h_, int c_, int w_) {
     if (F >= WAVE) {
       // a chunk lies inside one list: its state is a wave-uniform readlane
+      // of (head | cnt << 10 | want << 21) and of the 64-bit position
+      const uint32_t pk = (uint32_t)h_ | ((uint32_t)c_ << 10) | ((uint32_t)w_ << 21);
 #pragma unroll
       for (int u = 0; u < CHUNKS; ++u) {
         const int lr = (u * WAVE) / F;
         if (lr < Ka) {
-          const int hl = __builtin_amdgcn_readlane(h_, lr);
-          const int cl = __builtin_amdgcn_readlane(c_, lr);
-          const int wl = __builtin_amdgcn_readlane(w_, lr);
-          const int64_t cu = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(cu_ >> 32), lr) << 32) |
-                                       (uint32_t)__builtin_amdgcn_readlane((int)cu_, lr));
-          const int i = ((u * WAVE + lane) - hl) & (F - 1);
-          if (i >= cl && i < wl) dma_elem<DT>(a, cu + i, win + u * WAVE, WIN);
+          const uint32_t ps = (uint32_t)__builtin_amdgcn_readlane((int)pk, lr);
+          const int hl = (int)(ps & 1023u), cl = (int)((ps >> 10) & 2047u), wl = (int)(ps >> 21);
+          if (cl < wl) {
+            const int64_t cu = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(cu_ >> 32), lr) << 32) |
+                                         (uint32_t)__builtin_amdgcn_readlane((int)cu_, lr));
+            const int i = ((u * WAVE + lane) - hl) & (F - 1);
+            if (i >= cl && i < wl) dma_elem<DT>(a, cu, (uint32_t)i, win + u * WAVE, WIN);
+          }
         }
       }
     } else {
@@ -165,7 +209,7 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
         const int wl = __shfl(w_, l, WAVE);
         const int64_t cu = __shfl(cu_, l, WAVE);
         const int i = (slot - hl) & (F - 1);  // logical index of this slot
-        if (l < Ka && i >= cl && i < wl) dma_elem<DT>(a, cu + i, win + u * WAVE, WIN);
+        if (l < Ka && i >= cl && i < wl) dma_elem<DT>(a, cu, (uint32_t)i, win + u * WAVE, WIN);
       }
     }
   };
@@ -186,35 +230,40 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
 
     // ---- frontier y = min over full windows of the last element, tag (elem, list)
     const bool full = live && want == F;
-    E y = full ? lget<E>(win, WIN, lane * F + ((head + F - 1) & (F - 1))) : Limits<E>::hi();
-    int ky = full ? lane : WAVE;
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) {
-      const E oy = wxor(y, d);
-      const int oky = __shfl_xor(ky, d, WAVE);
-      const bool take = oky < WAVE && (ky == WAVE || lt(oy, y) || (!lt(y, oy) && oky < ky));
-      y = take ? oy : y;
-      ky = take ? oky : ky;
-    }
-    int sk;
+    E y;
+    int ky;
+    frontier_min<E>(full ? lget<E>(win, WIN, lane * F + ((head + F - 1) & (F - 1))) : Limits<E>::hi(),
+                    full ? lane : WAVE, Kp, y, ky);
+    // ---- elements each list takes: its window prefix ordered before (y, ky)
+    int sk = 0;
     bool gallop = false;
     int64_t glen = 0;
     if (ky == WAVE) {
-      sk = want;  // everything left is in the windows
+      sk = want;  // no full window: everything left is in the windows
     } else {
-      const bool ne = live && want > 0 && lane != ky;
-      E z = ne ? lget<E>(win, WIN, lane * F + head) : Limits<E>::hi();
-      int kz = ne ? lane : WAVE;
-#pragma unroll
-      for (int d = 1; d < WAVE; d <<= 1) {
-        const E oz = wxor(z, d);
-        const int okz = __shfl_xor(kz, d, WAVE);
-        const bool take = okz < WAVE && (kz == WAVE || lt(oz, z) || (!lt(z, oz) && okz < kz));
-        z = take ? oz : z;
-        kz = take ? okz : kz;
+      if (live && want > 0) {
+        if (lane == ky) {
+          sk = F;
+        } else {
+          int l = 0, h = want;
+          while (l < h) {
+            const int mid = (l + h) >> 1;
+            const E v = lget<E>(win, WIN, lane * F + ((head + mid) & (F - 1)));
+            const bool tk = lane < ky ? !lt(y, v) : lt(v, y);
+            l = tk ? mid + 1 : l;
+            h = tk ? h : mid;
+          }
+          sk = l;
+        }
       }
-      gallop = kz == WAVE || lt(y, z) || (!lt(z, y) && ky < kz);
+      // gallop when no other list takes anything: list ky's run continues
+      // past its window up to the smallest head z of the other lists
+      gallop = __ballot(live && lane != ky && sk > 0) == 0;
       if (gallop) {
+        E z;
+        int kz;
+        const bool ne = live && want > 0 && lane != ky;
+        frontier_min<E>(ne ? lget<E>(win, WIN, lane * F + head) : Limits<E>::hi(), ne ? lane : WAVE, Kp, z, kz);
         // run of list ky (from cur + F) that precedes z
         const int64_t base = __shfl(cur, ky, WAVE);
         int64_t glo = F, ghi = __shfl(endp, ky, WAVE) - base;
@@ -244,23 +293,6 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
         }
         glen = glo;
         sk = 0;
-      } else {
-        sk = 0;
-        if (live && want > 0) {
-          if (lane == ky) {
-            sk = F;
-          } else {
-            int l = 0, h = want;
-            while (l < h) {
-              const int mid = (l + h) >> 1;
-              const E v = lget<E>(win, WIN, lane * F + ((head + mid) & (F - 1)));
-              const bool tk = lane < ky ? !lt(y, v) : lt(v, y);
-              l = tk ? mid + 1 : l;
-              h = tk ? h : mid;
-            }
-            sk = l;
-          }
-        }
       }
     }
 
@@ -370,7 +402,7 @@ __global__ __launch_bounds__(WM_WAVES* WAVE, 4) void k_wmerge(RoundArgs a) {
   using E = typename DType<DT>::E;
   extern __shared__ uint32_t lds[];
   const int lane = threadIdx.x & (WAVE - 1);
-  const int wv = threadIdx.x / WAVE;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);  // wave-uniform
   const int64_t merger = (int64_t)blockIdx.x * WM_WAVES + wv;
   if (merger >= a.tasks * a.m) return;  // whole wavefront leaves; no barriers in this kernel
   uint32_t* my = lds + (size_t)wv * wmerge_lds_words_per_wave(E::W, WIN);
@@ -386,7 +418,7 @@ __global__ __launch_bounds__(WM_WAVES* WAVE, 4) void k_wmerge_persistent(RoundAr
   using E = typename DType<DT>::E;
   extern __shared__ uint32_t lds[];
   const int lane = threadIdx.x & (WAVE - 1);
-  const int wv = threadIdx.x / WAVE;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);  // wave-uniform
   uint32_t* my = lds + (size_t)wv * wmerge_lds_words_per_wave(E::W, WIN);
   for (;;) {
     unsigned int it = 0;
