@@ -1,0 +1,79 @@
+"""Harness parity with the reference's scripts: the testMM command line
+(src/testMM.cu:133-218: `./testMM N` prints "avg min max" then "SORTED!"),
+the size sweeps of scripts/allSizes.sh (2^k plus a random fraction of 2^k,
+k = 10..26) and scripts/powersOfTwoInput.sh, and the repeat run of
+scripts/raceConditions.sh (many sorts at N = 2^20 + 9000, all identical)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TESTMM = os.path.join(ROOT, "gpu-mms_amd", "cli", "testMM")
+
+
+def _to_dev(a, dev):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def test_testmm_cli_output(dev):
+    if not os.path.exists(TESTMM):
+        pytest.fail("testMM is not built (make -C gpu-mms_amd/csrc)")
+    print("testMM 1000000 2", flush=True)
+    r = subprocess.run([TESTMM, "1000000", "2"], capture_output=True, text=True, timeout=120)
+    lines = r.stdout.strip().splitlines()
+    assert r.returncode == 0, r.stdout + r.stderr
+    avg, mn, mx = (float(x) for x in lines[0].split())
+    assert 0 < mn <= avg <= mx
+    assert lines[1] == "SORTED!"
+    assert lines[2] == "MATCHES std::sort"
+
+
+def test_testmm_cli_usage():
+    if not os.path.exists(TESTMM):
+        pytest.fail("testMM is not built (make -C gpu-mms_amd/csrc)")
+    r = subprocess.run([TESTMM], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "Usage" in r.stdout
+
+
+def test_size_sweep(gmms, oracle, dev):
+    # 2^k + rand * 2^k / 32768 for k = 10..24, two draws each (allSizes.sh shape)
+    rng = np.random.default_rng(0x5EE9)
+    for k in range(10, 25):
+        for _ in range(2):
+            n = (1 << k) + int(rng.integers(0, 32768)) * (1 << k) // 32768
+            print(f"sweep n={n}", flush=True)
+            a = oracle.gen_i32(n, n, 32)
+            got = gmms.sort(_to_dev(a, dev)).cpu().numpy()
+            assert np.array_equal(got, oracle.sort(a)), n
+
+
+def test_powers_of_two(gmms, oracle, dev):
+    for k in range(10, 26):
+        n = 1 << k
+        print(f"pow2 n=2^{k}", flush=True)
+        a = oracle.glibc_rand_list(0, n, 1024) if k <= 20 else oracle.gen_i32(k, n, 32)
+        got = gmms.sort(_to_dev(a, dev)).cpu().numpy()
+        assert np.array_equal(got, np.sort(a)), k
+
+
+def test_race_repeat(gmms, oracle, dev):
+    import torch
+
+    n = (1 << 20) + 9000
+    print("race repeat x100", flush=True)
+    a = oracle.glibc_rand_list(0, n, 1024)
+    want = _to_dev(oracle.sort(a), dev)
+    src = _to_dev(a, dev)
+    k = torch.empty_like(src)
+    ka = torch.empty_like(src)
+    ws = gmms.Workspace()
+    for it in range(100):
+        k.copy_(src)
+        alt = gmms.sort_inplace(k, ka, workspace=ws)
+        assert torch.equal(ka if alt else k, want), f"run {it}"
