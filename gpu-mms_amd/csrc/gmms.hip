@@ -22,16 +22,25 @@ using namespace gmms_host;
 namespace {
 
 template <int DT>
-int launch_basecase(void* keys, uint32_t* vals, int64_t n, bool cin, bool cout, hipStream_t s) {
+int launch_basecase(void* keys, uint32_t* vals, int64_t n, bool cin, bool cout, bool run4, hipStream_t s) {
   if (n <= 0) return GMMS_SUCCESS;
   StageScope scope(GMMS_STAGE_BASECASE, n, s);
   const int64_t tiles = (n + TILE - 1) / TILE;
   const dim3 grid((unsigned)((tiles + BASE_WAVES - 1) / BASE_WAVES));
   const dim3 block(BASE_WAVES * WAVE);
-  if (cin && cout) hipLaunchKernelGGL((k_basecase<DT, true, true>), grid, block, 0, s, keys, vals, n);
-  else if (cin) hipLaunchKernelGGL((k_basecase<DT, true, false>), grid, block, 0, s, keys, vals, n);
-  else if (cout) hipLaunchKernelGGL((k_basecase<DT, false, true>), grid, block, 0, s, keys, vals, n);
-  else hipLaunchKernelGGL((k_basecase<DT, false, false>), grid, block, 0, s, keys, vals, n);
+  if (run4) {
+    // the sort's own base case: maps the caller's bits in, and out only if no round follows
+    if (cout) hipLaunchKernelGGL((k_basecase<DT, true, true, true>), grid, block, 0, s, keys, vals, n);
+    else hipLaunchKernelGGL((k_basecase<DT, true, false, true>), grid, block, 0, s, keys, vals, n);
+  } else if (cin && cout) {
+    hipLaunchKernelGGL((k_basecase<DT, true, true>), grid, block, 0, s, keys, vals, n);
+  } else if (cin) {
+    hipLaunchKernelGGL((k_basecase<DT, true, false>), grid, block, 0, s, keys, vals, n);
+  } else if (cout) {
+    hipLaunchKernelGGL((k_basecase<DT, false, true>), grid, block, 0, s, keys, vals, n);
+  } else {
+    hipLaunchKernelGGL((k_basecase<DT, false, false>), grid, block, 0, s, keys, vals, n);
+  }
   HIP_OK(hipGetLastError());
   return GMMS_SUCCESS;
 }
@@ -171,13 +180,13 @@ template <int DT>
 int run_sort(void* k0, void* k1, void* v0, void* v1, int64_t n, const Resolved& r, void* ws, size_t wsb,
              hipStream_t s, int* alt) {
   int rounds = 0;
-  for (int64_t L = TILE; L < n; L *= r.K) ++rounds;
-  int st = launch_basecase<DT>(k0, (uint32_t*)v0, n, true, rounds == 0, s);
+  for (int64_t L = r.run; L < n; L *= r.K) ++rounds;
+  int st = launch_basecase<DT>(k0, (uint32_t*)v0, n, true, rounds == 0, r.run > TILE, s);
   if (st) return st;
   void* kb[2] = {k0, k1};
   void* vb[2] = {v0, v1};
   int cur = 0, done = 0;
-  for (int64_t L = TILE; L < n; L *= r.K, ++done) {
+  for (int64_t L = r.run; L < n; L *= r.K, ++done) {
     st = run_round<DT>(kb[cur], vb[cur], kb[cur ^ 1], vb[cur ^ 1], n, L, r, done == rounds - 1, ws, wsb, s);
     if (st) return st;
     cur ^= 1;
@@ -281,6 +290,7 @@ int gmms_resolve_params(int dtype, const gmms_params* in, gmms_params* out) {
   out->mergers = r.mergers;
   out->prefetch = r.D;
   out->slack = (int32_t)r.slack;
+  out->base_run = r.run;
   return GMMS_SUCCESS;
 }
 
@@ -297,6 +307,17 @@ int gmms_num_rounds(int64_t n, int K, int* rounds) {
   int r = 0;
   for (int64_t L = TILE; L < n; L *= K) ++r;
   *rounds = r;
+  return GMMS_SUCCESS;
+}
+
+int gmms_sort_rounds(int dtype, int64_t n, const gmms_params* p, int* rounds) {
+  if (!valid_dtype(dtype) || !rounds || n < 0) return GMMS_ERROR_INVALID_VALUE;
+  Resolved r;
+  int st = resolve(dtype, p, &r, true);
+  if (st) return st;
+  int k = 0;
+  for (int64_t L = r.run; L < n; L *= r.K) ++k;
+  *rounds = k;
   return GMMS_SUCCESS;
 }
 
@@ -386,7 +407,8 @@ int gmms_squaresort(int dtype, void* keys, void* vals, int64_t n, int to_interna
   if (!keys || has_vals(dtype) != (vals != nullptr)) return GMMS_ERROR_INVALID_VALUE;
   hipStream_t s = (hipStream_t)stream;
   return dispatch(dtype, [&](auto dt) {
-    return launch_basecase<decltype(dt)::value>(keys, (uint32_t*)vals, n, to_internal != 0, to_external != 0, s);
+    return launch_basecase<decltype(dt)::value>(keys, (uint32_t*)vals, n, to_internal != 0, to_external != 0, false,
+                                                s);
   });
 }
 

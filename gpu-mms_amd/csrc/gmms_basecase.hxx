@@ -114,9 +114,42 @@ constexpr int BASE_WAVES = 4;                   // waves (tiles) per workgroup
 constexpr int XPOSE_STRIDE = TPL + 1;            // bank padding
 constexpr int XPOSE_WORDS = WAVE * XPOSE_STRIDE; // per wave per word plane
 
+// Exchange with another wavefront of the workgroup through LDS: lane l,
+// register r receives the partner wavefront's element at lane 63-l, register
+// 15-r (MIRROR: the flip stage across two 1024-blocks) or at lane l,
+// register r (a half-cleaner at distance 1024), and keeps the minimum if LOW.
+// `buf` holds BASE_WAVES * W * XPOSE_WORDS words; every wavefront of the
+// workgroup calls this together (two workgroup barriers).
+template <bool MIRROR, class E>
+__device__ __forceinline__ void wave_exchange(E (&v)[TPL], uint32_t* buf, int wave, int partner, bool low, int lane) {
+  constexpr int W = E::W;
+  __syncthreads();  // the previous use of buf is over
+  uint32_t* mine = buf + wave * W * XPOSE_WORDS;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+#pragma unroll
+    for (int r = 0; r < TPL; ++r) mine[w * XPOSE_WORDS + lane * XPOSE_STRIDE + r] = word(v[r], w);
+  }
+  __syncthreads();
+  const uint32_t* theirs = buf + partner * W * XPOSE_WORDS;
+  const int pl = MIRROR ? WAVE - 1 - lane : lane;
+  const uint32_t hsel = low ? 0u : 0xFFFFFFFFu;
+#pragma unroll
+  for (int r = 0; r < TPL; ++r) {
+    const int pr = MIRROR ? TPL - 1 - r : r;
+    uint32_t wd[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) wd[w] = theirs[w * XPOSE_WORDS + pl * XPOSE_STRIDE + pr];
+    v[r] = lane_ce(v[r], from_words<E>(wd), low, hsel);
+  }
+}
+
 // In-place base case over keys[0..n): CONV_IN maps raw -> internal on load,
-// CONV_OUT maps internal -> raw on store.
-template <int DT, bool CONV_IN, bool CONV_OUT>
+// CONV_OUT maps internal -> raw on store. RUN4: the workgroup's four sorted
+// tiles are merged into one sorted run of 4096 (two more merge levels; the
+// flip and distance-1024 stages go through LDS between wavefronts, the rest
+// are the in-wavefront stages of merge_level).
+template <int DT, bool CONV_IN, bool CONV_OUT, bool RUN4 = false>
 __global__ __launch_bounds__(BASE_WAVES * WAVE) void k_basecase(void* keys, uint32_t* vals, int64_t n) {
   using E = typename DType<DT>::E;
   __shared__ uint32_t xp[BASE_WAVES * E::W * XPOSE_WORDS];
@@ -124,8 +157,11 @@ __global__ __launch_bounds__(BASE_WAVES * WAVE) void k_basecase(void* keys, uint
   const int wave = threadIdx.x / WAVE;
   const int64_t tile = (int64_t)blockIdx.x * BASE_WAVES + wave;
   const int64_t base = tile * TILE;
-  if (base >= n) return;  // per-wave work only: no workgroup barrier below
-  const int cnt = (int)((n - base) < TILE ? (n - base) : TILE);
+  // without RUN4 there is no workgroup barrier below and an idle wave leaves;
+  // with it, a wave past the end sorts a tile of maxima and stores nothing
+  if (!RUN4 && base >= n) return;
+  const int64_t rest = n - base;
+  const int cnt = rest <= 0 ? 0 : (int)(rest < TILE ? rest : TILE);
 
   E v[TPL];
 #pragma unroll
@@ -135,6 +171,21 @@ __global__ __launch_bounds__(BASE_WAVES * WAVE) void k_basecase(void* keys, uint
   }
 
   sort_tile(v, lane);
+
+  if constexpr (RUN4) {
+    static_assert(BASE_WAVES == 4, "a run of 4096 is four tiles");
+    // 1024 -> 2048: flip across wavefront pairs, then the in-wavefront cleaners
+    wave_exchange<true>(v, xp, wave, wave ^ 1, (wave & 1) == 0, lane);
+    lane_cleaners<6>(v, lane);
+    clean_regs(v);
+    // 2048 -> 4096: flip across the two halves, the cleaner at distance 1024,
+    // then the in-wavefront cleaners
+    wave_exchange<true>(v, xp, wave, 3 - wave, wave < 2, lane);
+    wave_exchange<false>(v, xp, wave, wave ^ 1, (wave & 1) == 0, lane);
+    lane_cleaners<6>(v, lane);
+    clean_regs(v);
+    __syncthreads();  // xp is reused by the store transpose below
+  }
 
   // transpose through LDS: lane l's registers hold ranks 16l..16l+15
   uint32_t* my = xp + wave * E::W * XPOSE_WORDS;

@@ -25,6 +25,7 @@ struct Resolved {
   int K, PL, D, mergers;
   int64_t slack;
   int kernel;  // GMMS_MERGE_* : wavefront tile, pipelined heap, workgroup tile
+  int run;     // sorted run length after the base case (1024 or 4096)
 };
 
 enum { GMMS_MERGE_WAVE = 0, GMMS_MERGE_HEAP = 1, GMMS_MERGE_TILE = 2 };
@@ -109,6 +110,8 @@ inline int resolve(int dtype, const gmms_params* p, Resolved* r, bool need_devic
   const int per_cu = kern == GMMS_MERGE_HEAP ? 2 : (kern == GMMS_MERGE_TILE ? 3 : wm_resident_per_cu(W));
   r->mergers = q.mergers ? q.mergers : per_cu * (need_device ? cu_count() : 256);
   r->slack = q.slack;
+  if (q.base_run != 0 && q.base_run != TILE && q.base_run != TILE * BASE_WAVES) return GMMS_ERROR_INVALID_VALUE;
+  r->run = q.base_run ? q.base_run : TILE * BASE_WAVES;
   return GMMS_SUCCESS;
 }
 
@@ -134,7 +137,7 @@ inline int64_t slack_for(const Resolved& r, int64_t n, int64_t L, int m) {
 
 inline size_t ws_bytes_for(int64_t n, const Resolved& r) {
   size_t need = 0;
-  for (int64_t L = TILE; L < n; L *= r.K) {
+  for (int64_t L = r.run; L < n; L *= r.K) {
     const int64_t TK = (int64_t)r.K * L;
     const int64_t tasks = (n + TK - 1) / TK;
     const int m = mergers_per_task(n, L, r.K, r.mergers);

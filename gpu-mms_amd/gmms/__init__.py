@@ -65,7 +65,8 @@ class Params(ctypes.Structure):
         ("mergers", ctypes.c_int32),
         ("prefetch", ctypes.c_int32),
         ("slack", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("base_run", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 
@@ -148,6 +149,7 @@ def lib() -> ctypes.CDLL:
         "gmms_resolve_params": (i32, [i32, P, P]),
         "gmms_element_size": (i32, [i32, ctypes.POINTER(sz), ctypes.POINTER(sz)]),
         "gmms_num_rounds": (i32, [i64, i32, ctypes.POINTER(i32)]),
+        "gmms_sort_rounds": (i32, [i32, i64, P, ctypes.POINTER(i32)]),
         "gmms_workspace_size": (i32, [i32, i64, P, ctypes.POINTER(sz)]),
         "gmms_round_workspace_size": (i32, [i32, i64, i64, P, ctypes.POINTER(sz)]),
         "gmms_sort": (i32, [i32, vp, vp, vp, vp, i64, P, vp, sz, vp, ctypes.POINTER(i32)]),
@@ -217,9 +219,13 @@ def default_params() -> Params:
 
 
 def make_params(K: int = 0, mergers: int = 0, prefetch: int = 0, slack: int = 0,
-                pipeline: Optional[int] = None) -> Params:
-    """pipeline=1: pipelined minBlockHeap merge; pipeline=0: tile merge."""
+                pipeline: Optional[int] = None, base_run: int = 0) -> Params:
+    """pipeline=1: pipelined minBlockHeap merge; pipeline=0: tile merge.
+    base_run: 1024 (the reference's tile) or 4096 sorted elements per base
+    run; 0: auto."""
     p = default_params()
+    if base_run:
+        p.base_run = base_run
     if pipeline is not None:
         p.pipeline = int(pipeline)
     if K:
@@ -241,8 +247,16 @@ def resolve_params(dtype: int, params: Optional[Params] = None) -> Params:
 
 
 def num_rounds(n: int, K: int = 64) -> int:
+    """Rounds of the reference's schedule (lists start at 1024)."""
     r = ctypes.c_int()
     _check(lib().gmms_num_rounds(n, K, ctypes.byref(r)), "gmms_num_rounds")
+    return r.value
+
+
+def sort_rounds(dtype: int, n: int, params: Optional[Params] = None) -> int:
+    """Rounds gmms_sort runs for n elements with these knobs."""
+    r = ctypes.c_int()
+    _check(lib().gmms_sort_rounds(dtype, n, _pp(params), ctypes.byref(r)), "gmms_sort_rounds")
     return r.value
 
 

@@ -68,7 +68,10 @@ typedef struct gmms_params {
   int32_t mergers;   /* target mergers per round when tasks are few; 0: auto */
   int32_t prefetch;  /* blocks prefetched per list into LDS (1..8)           */
   int32_t slack;     /* partition tolerance in elements; 0: auto             */
-  int32_t reserved[3];
+  int32_t base_run;  /* sorted run length the base case leaves: 1024 (one
+                        wavefront tile, the reference's M) or 4096 (four tiles
+                        merged in a workgroup); 0: auto (4096)               */
+  int32_t reserved[2];
 } gmms_params;
 
 /* Library version string, e.g. "gpu-mms-amd 0.1.0 gfx950". */
@@ -86,8 +89,13 @@ int gmms_resolve_params(int dtype, const gmms_params* in, gmms_params* out);
 /* Bytes per key and per value (0 for key-only dtypes) of a dtype. */
 int gmms_element_size(int dtype, size_t* key_bytes, size_t* value_bytes);
 /* Number of merge rounds for n elements: iterations of the reference's
- * `for (listSize = M; listSize < N; listSize *= K)` (multimergesort.hxx:87). */
+ * `for (listSize = M; listSize < N; listSize *= K)` (multimergesort.hxx:87),
+ * M = 1024. */
 int gmms_num_rounds(int64_t n, int K, int* rounds);
+/* Merge rounds gmms_sort runs for n elements with these knobs: the same loop
+ * from listSize = the resolved base_run (so one round fewer than
+ * gmms_num_rounds whenever the 4096-element base runs save one). */
+int gmms_sort_rounds(int dtype, int64_t n, const gmms_params* p, int* rounds);
 
 /* Device workspace (partition cuts) that gmms_sort needs for n elements.
  * The reference cudaMalloc's its pivot array inside every call

@@ -46,6 +46,24 @@ def test_element_sizes(gmms):
     assert L.gmms_element_size(42, ctypes.byref(kb), ctypes.byref(vb)) == gmms.ERROR_INVALID_VALUE
 
 
+def test_base_run_and_sort_rounds(gmms):
+    assert gmms.default_params().base_run == 0
+    assert gmms.resolve_params(gmms.INT32).base_run == 4096
+    assert gmms.resolve_params(gmms.INT32, gmms.make_params(base_run=1024)).base_run == 1024
+    with pytest.raises(gmms.GmmsError):
+        gmms.resolve_params(gmms.INT32, gmms.make_params(base_run=2048))
+    for n in (0, 1, 4096, 4097, 1 << 20, 1 << 28, 10**8 + 17):
+        for K in (4, 16, 64):
+            for run in (1024, 4096):
+                r, L = 0, run
+                while L < n:
+                    r, L = r + 1, L * K
+                assert gmms.sort_rounds(gmms.INT32, n, gmms.make_params(K=K, base_run=run)) == r
+    # 2^28 with K = 16: the 4096-element runs save the fifth round
+    assert gmms.num_rounds(1 << 28, 16) == 5
+    assert gmms.sort_rounds(gmms.INT32, 1 << 28, gmms.make_params(K=16)) == 4
+
+
 def test_num_rounds_matches_reference_loop(gmms, oracle):
     for n in (0, 1, 1024, 1025, 1 << 20, 10**8 + 17, 1 << 28, 1 << 33):
         for K in (2, 8, 64):

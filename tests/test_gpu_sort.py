@@ -196,7 +196,7 @@ def test_multimergesort_dropin_returns_result_buffer(gmms, oracle, dev):
         inp = _to_dev(a, dev)
         out = torch.empty_like(inp)
         res = gmms.multimergesort(inp, out, thread_blocks=482)
-        assert gmms.num_rounds(n, 64) == rounds
+        assert gmms.sort_rounds(gmms.INT32, n) == rounds
         assert (res is out) == (rounds % 2 == 1)
         assert np.array_equal(_host(res), oracle.sort(a))
 
@@ -220,3 +220,24 @@ def test_repeat_is_bit_identical(gmms, oracle, dev):
     for _ in range(20):
         got = gmms.sort(src, params=gmms.make_params(mergers=997))
         assert np.array_equal(_host(got), want)
+
+
+@pytest.mark.parametrize("run", [1024, 4096])
+def test_base_run_lengths(gmms, oracle, dev, run):
+    # the base case leaves runs of one tile (the reference's M) or of four
+    # tiles merged in a workgroup; ragged ends leave some wavefronts idle
+    import torch
+
+    p = gmms.make_params(base_run=run)
+    for n in (1, 1000, 1024, 1025, 3 * 1024 + 7, 4095, 4096, 4097, 5000, 9000, 65541, 300_001, (1 << 20) + 3):
+        print(f"base_run={run} n={n}", flush=True)
+        a = oracle.gen_i32(7 * n + run, n, 32)
+        assert np.array_equal(_gpu_sort(gmms, a, dev, base_run=run), oracle.sort(a)), n
+    k, v = oracle.gen_pairs(5, 70_001, 3, 1.1, 1 << 20)
+    ks, vs = gmms.sort(torch.from_numpy(k.view(np.int64)).to(dev), torch.from_numpy(v.view(np.int32)).to(dev),
+                       params=p)
+    wk, wv = oracle.sort_pairs(k, v)
+    assert np.array_equal(_host(ks).view(np.uint64), wk) and np.array_equal(_host(vs).view(np.uint32), wv)
+    x = oracle.gen_u64(6, 50_003).view(np.float64).copy()
+    got = _host(gmms.sort(torch.from_numpy(x).to(dev), params=p))
+    assert np.array_equal(got.view(np.uint64), oracle.sort(x).view(np.uint64))
