@@ -368,7 +368,8 @@ def main():
     K = resolved.kway
     kernel_name = {0: "k_wmerge (wavefront tile merge)", 1: "k_merge (pipelined minBlockHeap)",
                    2: "k_tmerge (workgroup tile merge)"}[resolved.pipeline]
-    rounds = gmms.num_rounds(n, K)
+    rounds = gmms.sort_rounds(gmms.INT32, n, params)  # rounds this build runs
+    rounds_ref = gmms.num_rounds(n, K)  # ceil(log_K(N/1024)): the reference's schedule
     # keep an untouched copy of input 0 for the bit-exact check
     check_src = base.clone()
 
@@ -430,8 +431,13 @@ def main():
     oracle.lib.oracle_psort_i32(want.ctypes.data, n, threads)
     bit_exact = bool(np.array_equal(got, want))
 
-    formula_bytes = 2.0 * n * 4 * (1 + rounds)
+    # the roofline's byte formula 2*N*4*(1 + ceil(log_K(N/1024))) with the K
+    # that ran, and the bytes of the passes this build actually makes
+    # (base case + rounds; its base runs are 4096 long, so it may make fewer)
+    formula_bytes = 2.0 * n * 4 * (1 + rounds_ref)
+    pass_bytes = 2.0 * n * 4 * (1 + rounds)
     sort_gbps = formula_bytes / (ms_per_step * 1e-3) / 1e9
+    pass_gbps = pass_bytes / (ms_per_step * 1e-3) / 1e9
 
     log(f"bit-exact check done: {bit_exact}")
     cpu = None
@@ -473,7 +479,8 @@ def main():
         "dtype": "int32",
         "data": f"synthetic: mt19937_64 seed {seed:#x}, key=(int32)(u64>>32); step i XORs a constant",
         "config": {"workload": "plain sort, N=2^28 int32 uniform over the full int32 range",
-                   "n": n, "K": K, "rounds": rounds, "merge_kernel": resolved.pipeline,
+                   "n": n, "K": K, "rounds": rounds, "rounds_reference_schedule": rounds_ref,
+                   "base_run": resolved.base_run, "merge_kernel": resolved.pipeline,
                    "mergers": resolved.mergers},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2) if achieved else None,
                      "peak": PUBLISHED_HBM_PEAK_GBPS, "unit": "GB/s",
@@ -484,9 +491,13 @@ def main():
         "cpu_baseline": cpu,
         "bit_exact_vs_std_sort": bit_exact,
         "stream_copy_peak_GBps": round(copy_peak, 1),
-        "sort_roofline": {"bytes": formula_bytes, "achieved_GBps": round(sort_gbps, 1),
+        "sort_roofline": {"formula": "2*N*4*(1+ceil(log_K(N/1024)))", "bytes": formula_bytes,
+                          "achieved_GBps": round(sort_gbps, 1),
                           "frac_of_copy_peak": round(sort_gbps / copy_peak, 4),
-                          "frac_of_published_peak": round(sort_gbps / PUBLISHED_HBM_PEAK_GBPS, 4)},
+                          "frac_of_published_peak": round(sort_gbps / PUBLISHED_HBM_PEAK_GBPS, 4),
+                          "passes_made": 1 + rounds, "pass_bytes": pass_bytes,
+                          "pass_GBps": round(pass_gbps, 1),
+                          "pass_frac_of_copy_peak": round(pass_gbps / copy_peak, 4)},
         "stage_ms_avg": {k: round(sum(v) / len(v), 4) for k, v in per_kind.items()},
         "merge_round_ms": per_round,
         "batched": batch,
