@@ -234,7 +234,12 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
     int ky;
     frontier_min<E>(full ? lget<E>(win, WIN, lane * F + ((head + F - 1) & (F - 1))) : Limits<E>::hi(),
                     full ? lane : WAVE, Kp, y, ky);
-    // ---- elements each list takes: its window prefix ordered before (y, ky)
+    // ---- elements each list takes: its window prefix <= y. Every element
+    // still outside the windows is >= y (y is the smallest last element of a
+    // full window), so these S elements are the next S outputs. Equal
+    // elements are equal bit for bit, so their order across lists does not
+    // matter: with many duplicates every list takes its copies of y at once
+    // instead of one list per step.
     int sk = 0;
     bool gallop = false;
     int64_t glen = 0;
@@ -249,7 +254,7 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
           while (l < h) {
             const int mid = (l + h) >> 1;
             const E v = lget<E>(win, WIN, lane * F + ((head + mid) & (F - 1)));
-            const bool tk = lane < ky ? !lt(y, v) : lt(v, y);
+            const bool tk = !lt(y, v);  // v <= y
             l = tk ? mid + 1 : l;
             h = tk ? h : mid;
           }
@@ -258,6 +263,7 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
       }
       // gallop when no other list takes anything: list ky's run continues
       // past its window up to the smallest head z of the other lists
+      // (elements <= z)
       gallop = __ballot(live && lane != ky && sk > 0) == 0;
       if (gallop) {
         E z;
@@ -275,7 +281,7 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
             bool before = true;
             if (p < ghi) {
               const E v = gload<DT, false>(a.in_keys, a.in_vals, base + p);
-              before = lt(v, z) || (!lt(z, v) && ky < kz);
+              before = !lt(z, v);  // v <= z
             }
             const unsigned long long nb = __ballot(!before && p < ghi);
             if (nb == 0) {
