@@ -76,19 +76,30 @@ __device__ __forceinline__ bool le(const E32& a, const E32& b) { return a.k <= b
 __device__ __forceinline__ bool le(const E64& a, const E64& b) { return a.k <= b.k; }
 __device__ __forceinline__ bool le(const P96& a, const P96& b) { return !lt(b, a); }
 
-template <class E> __device__ __forceinline__ E emin(const E& a, const E& b) { return lt(b, a) ? b : a; }
-template <class E> __device__ __forceinline__ E emax(const E& a, const E& b) { return lt(b, a) ? a : b; }
+// c ? a : b field by field (a select of a whole padded struct can end up in
+// scratch memory)
+__device__ __forceinline__ E32 sel(bool c, const E32& a, const E32& b) { return {c ? a.k : b.k}; }
+__device__ __forceinline__ E64 sel(bool c, const E64& a, const E64& b) { return {c ? a.k : b.k}; }
+__device__ __forceinline__ P96 sel(bool c, const P96& a, const P96& b) { return {c ? a.k : b.k, c ? a.v : b.v}; }
+
+template <class E> __device__ __forceinline__ E emin(const E& a, const E& b) { return sel(lt(b, a), b, a); }
+template <class E> __device__ __forceinline__ E emax(const E& a, const E& b) { return sel(lt(b, a), a, b); }
 template <> __device__ __forceinline__ E32 emin(const E32& a, const E32& b) { return {a.k < b.k ? a.k : b.k}; }
 template <> __device__ __forceinline__ E32 emax(const E32& a, const E32& b) { return {a.k < b.k ? b.k : a.k}; }
 
-// compare-exchange: a <- min, b <- max
+// compare-exchange: a <- min, b <- max (one comparison for multi-word
+// elements; min/max instructions for 32-bit ones)
 template <class E> __device__ __forceinline__ void cswap(E& a, E& b) {
-  E lo = emin(a, b), hi = emax(a, b);
+  const bool c = lt(b, a);
+  const E lo = sel(c, b, a), hi = sel(c, a, b);
   a = lo;
   b = hi;
 }
-
-template <class E> __device__ __forceinline__ E sel(bool c, const E& a, const E& b) { return c ? a : b; }
+template <> __device__ __forceinline__ void cswap(E32& a, E32& b) {
+  const E32 lo = emin(a, b), hi = emax(a, b);
+  a = lo;
+  b = hi;
+}
 
 // Cross-lane compare-exchange, one side: the lower lane of a pair keeps
 // min(x, p), the upper keeps max(x, p). hsel is 0 on lower lanes and all
@@ -102,7 +113,10 @@ __device__ __forceinline__ uint32_t med3_u32(uint32_t a, uint32_t b, uint32_t c)
 template <class E>
 __device__ __forceinline__ E lane_ce(const E& x, const E& p, bool low, uint32_t hsel) {
   (void)hsel;
-  return low ? emin(x, p) : emax(x, p);
+  // one comparison: the lower lane takes the partner if it is smaller, the
+  // upper lane if it is not (equal elements are identical either way)
+  const bool c = lt(p, x);
+  return sel(low == c, p, x);
 }
 template <>
 __device__ __forceinline__ E32 lane_ce(const E32& x, const E32& p, bool low, uint32_t hsel) {

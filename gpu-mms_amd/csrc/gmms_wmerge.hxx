@@ -79,7 +79,7 @@ __device__ __forceinline__ void frontier_min(E x, int k, int Kp, E& y, int& ky) 
       const E ox = wxor(x, d);
       const int ok = __shfl_xor(k, d, WAVE);
       const bool take = ok < WAVE && (k == WAVE || lt(ox, x) || (!lt(x, ox) && ok < k));
-      x = take ? ox : x;
+      x = sel(take, ox, x);
       k = take ? ok : k;
     }
   }
@@ -355,7 +355,7 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
 #pragma unroll
       for (int q = 0; q < NREG; ++q) {
         const E x = lget<E>(win, WIN, r * F + ((hd + i0 + q) & (F - 1)));
-        v[q] = i0 + q < tk ? x : Limits<E>::hi();
+        v[q] = sel(i0 + q < tk, x, Limits<E>::hi());
       }
       __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): window reads done before the DMA reuses the slots
       refill(cur2, head2, cnt2, want2);
