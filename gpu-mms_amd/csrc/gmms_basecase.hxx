@@ -208,28 +208,33 @@ __global__ __launch_bounds__(BASE_WAVES * WAVE) void k_basecase(void* keys, uint
   }
 }
 
-// Batched segments: tile descriptors instead of one array. A tile holds
-// `cnt` (1..TILE) elements of one segment at `off`; it is read from buffer 0
-// in the caller's bits and written, sorted, to buffer (flags & 1), in the
-// caller's bits when (flags & 2) (a segment of one tile has no merge round).
+// Batched segments: run descriptors instead of one array. A descriptor holds
+// `cnt` elements of one segment at `off` (1..TILE, or 1..4*TILE for the
+// workgroup-run variant); it is read from buffer 0 in the caller's bits and
+// written, sorted, to buffer (flags & 1), in the caller's bits when
+// (flags & 2) (a segment of one run has no merge round).
 struct TileDesc {
   int64_t off;
   int32_t cnt;
   int32_t flags;
 };
 
-template <int DT>
+// RUN4: one workgroup per descriptor of up to 4096 elements (four tiles of
+// one segment), merged into one sorted run as in k_basecase; otherwise one
+// wavefront per descriptor of up to 1024 elements.
+template <int DT, bool RUN4 = false>
 __global__ __launch_bounds__(BASE_WAVES* WAVE) void k_basecase_seg(void* k0, uint32_t* v0, void* k1, uint32_t* v1,
                                                                     const TileDesc* tiles, int64_t n_tiles) {
   using E = typename DType<DT>::E;
   __shared__ uint32_t xp[BASE_WAVES * E::W * XPOSE_WORDS];
   const int lane = threadIdx.x & (WAVE - 1);
   const int wave = threadIdx.x / WAVE;
-  const int64_t tile = (int64_t)blockIdx.x * BASE_WAVES + wave;
-  if (tile >= n_tiles) return;  // per-wave work only: no workgroup barrier below
+  const int64_t tile = RUN4 ? (int64_t)blockIdx.x : (int64_t)blockIdx.x * BASE_WAVES + wave;
+  if (tile >= n_tiles) return;  // RUN4: the whole workgroup leaves together
   const TileDesc td = tiles[tile];
-  const int64_t base = td.off;
-  const int cnt = td.cnt;
+  const int64_t base = RUN4 ? td.off + (int64_t)wave * TILE : td.off;
+  const int rest = RUN4 ? td.cnt - wave * TILE : td.cnt;
+  const int cnt = rest <= 0 ? 0 : (rest < TILE ? rest : TILE);
 
   E v[TPL];
 #pragma unroll
@@ -239,6 +244,17 @@ __global__ __launch_bounds__(BASE_WAVES* WAVE) void k_basecase_seg(void* k0, uin
   }
 
   sort_tile(v, lane);
+
+  if constexpr (RUN4) {
+    wave_exchange<true>(v, xp, wave, wave ^ 1, (wave & 1) == 0, lane);
+    lane_cleaners<6>(v, lane);
+    clean_regs(v);
+    wave_exchange<true>(v, xp, wave, 3 - wave, wave < 2, lane);
+    wave_exchange<false>(v, xp, wave, wave ^ 1, (wave & 1) == 0, lane);
+    lane_cleaners<6>(v, lane);
+    clean_regs(v);
+    __syncthreads();
+  }
 
   uint32_t* my = xp + wave * E::W * XPOSE_WORDS;
 #pragma unroll

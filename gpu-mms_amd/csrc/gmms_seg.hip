@@ -28,17 +28,17 @@ namespace {
 struct SegDesc {
   int64_t off;    // first element
   int64_t n;      // elements (> 0)
-  int64_t tile0;  // index of its first tile
-  int32_t flags;  // TileDesc flags of all its tiles
-  int32_t pad_;
+  int64_t tile0;  // index of its first run descriptor
+  int32_t flags;  // TileDesc flags of all its runs
+  int32_t run;    // run length: TILE or 4 * TILE
 };
 static_assert(sizeof(MergerInfo) == 48, "MergerInfo layout");
 static_assert(sizeof(TileDesc) == 16, "TileDesc layout");
 
 constexpr int SEG_SETUP_WAVES = 4;
 
-// One wavefront per segment writes the segment's tile descriptors; wavefront
-// 0 also clears the round counters.
+// One wavefront per segment writes the segment's run descriptors (runs of
+// sd.run elements); wavefront 0 also clears the round counters.
 __global__ __launch_bounds__(SEG_SETUP_WAVES* WAVE) void k_seg_setup(const SegDesc* segs, int64_t n_segs,
                                                                       TileDesc* tiles, unsigned int* queue,
                                                                       int n_queue) {
@@ -49,12 +49,12 @@ __global__ __launch_bounds__(SEG_SETUP_WAVES* WAVE) void k_seg_setup(const SegDe
   }
   if (w >= n_segs) return;
   const SegDesc sd = segs[w];
-  const int64_t nt = (sd.n + TILE - 1) / TILE;
+  const int64_t nt = (sd.n + sd.run - 1) / sd.run;
   for (int64_t t = lane; t < nt; t += WAVE) {
     TileDesc td;
-    td.off = sd.off + t * TILE;
-    const int64_t rest = sd.n - t * TILE;
-    td.cnt = (int32_t)(rest < TILE ? rest : TILE);
+    td.off = sd.off + t * sd.run;
+    const int64_t rest = sd.n - t * sd.run;
+    td.cnt = (int32_t)(rest < sd.run ? rest : sd.run);
     td.flags = sd.flags;
     tiles[sd.tile0 + t] = td;
   }
@@ -96,6 +96,7 @@ struct gmms_seg_plan {
   int R = 0;
   int slots = 0;  // resident mergers (for the persistent grid)
   int64_t n_seg = 0, elements = 0, extent = 0, tiles = 0;
+  int64_t runs4 = 0, runs1 = 0;  // run descriptors: 4096-element (first), 1024-element
   std::vector<int32_t> seg_K, seg_R;  // per segment (all, empties included)
   std::vector<SegRound> rounds;
   int64_t n_segs_dev = 0;  // non-empty segments
@@ -154,8 +155,10 @@ int plan_build(gmms_seg_plan* P, int dtype, const int64_t* offsets, int64_t n_se
     const int c = size_class(n);
     int K = forced ? r.K : (tune.K[c] ? tune.K[c] : SEG_DEFAULT_K);
     if (K > kmax) K = kmax;
+    // segments longer than one tile start from 4096-element runs
+    const int64_t run = n > TILE ? (int64_t)TILE * BASE_WAVES : TILE;
     int rs = 0;
-    for (int64_t L = TILE; L < n; L *= K) ++rs;
+    for (int64_t L = run; L < n; L *= K) ++rs;
     P->seg_K[s] = K;
     P->seg_R[s] = rs;
     seg_merger[s] = tune.merger[c];
@@ -170,21 +173,36 @@ int plan_build(gmms_seg_plan* P, int dtype, const int64_t* offsets, int64_t n_se
   P->extent = n_seg ? offsets[n_seg] : 0;
   P->n_segs_dev = nonempty;
 
-  // segment table
+  // segment table: the 4096-element run descriptors of all long segments
+  // come first, then the tile descriptors of the short ones
   std::vector<SegDesc> segs;
   segs.reserve(nonempty);
-  int64_t t0 = 0;
+  int64_t r4 = 0, r1 = 0;
+  for (int64_t s = 0; s < n_seg; ++s) {
+    const int64_t n = offsets[s + 1] - offsets[s];
+    if (n > TILE) r4 += (n + TILE * BASE_WAVES - 1) / (TILE * BASE_WAVES);
+    else if (n > 0) r1 += 1;
+  }
+  P->runs4 = r4;
+  P->runs1 = r1;
+  int64_t a4 = 0, a1 = r4;
   for (int64_t s = 0; s < n_seg; ++s) {
     const int64_t n = offsets[s + 1] - offsets[s];
     if (n == 0) continue;
     SegDesc d{};
     d.off = offsets[s];
     d.n = n;
-    d.tile0 = t0;
     const int rs = P->seg_R[s];
     d.flags = ((R - rs) & 1) | (rs == 0 ? 2 : 0);
+    if (n > TILE) {
+      d.run = TILE * BASE_WAVES;
+      d.tile0 = a4;
+      a4 += (n + d.run - 1) / d.run;
+    } else {
+      d.run = TILE;
+      d.tile0 = a1++;
+    }
     segs.push_back(d);
-    t0 += (n + TILE - 1) / TILE;
   }
 
   // rounds: round volume first (for the automatic merger size)
@@ -199,7 +217,7 @@ int plan_build(gmms_seg_plan* P, int dtype, const int64_t* offsets, int64_t n_se
     const int64_t n = offsets[s + 1] - offsets[s];
     const int rs = P->seg_R[s];
     const int K = P->seg_K[s];
-    int64_t L = TILE;
+    int64_t L = n > TILE ? (int64_t)TILE * BASE_WAVES : TILE;
     for (int rr = 0; rr < rs; ++rr, L *= K) {
       const int g = R - rs + rr;
       const int64_t TK = (int64_t)K * L;
@@ -268,14 +286,14 @@ int plan_build(gmms_seg_plan* P, int dtype, const int64_t* offsets, int64_t n_se
   P->off_queue = 0;
   P->off_table = align256(sizeof(unsigned int) * (size_t)std::max(R, 1));
   P->off_tiles = P->off_table + align256(P->host_bytes);
-  P->off_cuts = P->off_tiles + align256((size_t)tiles * sizeof(TileDesc));
+  P->off_cuts = P->off_tiles + align256((size_t)(P->runs4 + P->runs1) * sizeof(TileDesc));
   P->ws_bytes = P->off_cuts + align256((size_t)P->max_rows * MAX_K * sizeof(int64_t));
   return GMMS_SUCCESS;
 }
 
 int plan_launches(const gmms_seg_plan* P) {
   if (P->n_segs_dev == 0) return 0;
-  int n = 2;
+  int n = 1 + (P->runs4 > 0) + (P->runs1 > 0);
   for (const SegRound& sr : P->rounds) n += 1 + (sr.n_part > 0);
   return n;
 }
@@ -323,10 +341,17 @@ int seg_enqueue(const gmms_seg_plan* P, void* k0, void* k1, void* v0, void* v1, 
   }
   {
     std::unique_ptr<StageScope> sc(timed ? new StageScope(GMMS_STAGE_BASECASE, P->elements, s) : nullptr);
-    const int64_t grid = (P->tiles + BASE_WAVES - 1) / BASE_WAVES;
-    hipLaunchKernelGGL((k_basecase_seg<DT>), dim3((unsigned)grid), dim3(BASE_WAVES * WAVE), 0, s, k0,
-                       (uint32_t*)v0, k1, (uint32_t*)v1, (const TileDesc*)tiles, P->tiles);
-    HIP_OK(hipGetLastError());
+    if (P->runs4 > 0) {
+      hipLaunchKernelGGL((k_basecase_seg<DT, true>), dim3((unsigned)P->runs4), dim3(BASE_WAVES * WAVE), 0, s, k0,
+                         (uint32_t*)v0, k1, (uint32_t*)v1, (const TileDesc*)tiles, P->runs4);
+      HIP_OK(hipGetLastError());
+    }
+    if (P->runs1 > 0) {
+      const int64_t grid = (P->runs1 + BASE_WAVES - 1) / BASE_WAVES;
+      hipLaunchKernelGGL((k_basecase_seg<DT, false>), dim3((unsigned)grid), dim3(BASE_WAVES * WAVE), 0, s, k0,
+                         (uint32_t*)v0, k1, (uint32_t*)v1, (const TileDesc*)tiles + P->runs4, P->runs1);
+      HIP_OK(hipGetLastError());
+    }
   }
   void* kb[2] = {k0, k1};
   void* vb[2] = {v0, v1};

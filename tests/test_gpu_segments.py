@@ -151,9 +151,10 @@ def test_tuning_table_per_class(gmms, oracle, dev):
         gmms.seg_tuning_set(gmms.INT32, kk, mm)
         offs = _offsets([8000, 100_000, 1 << 20, 700_000, 5000])
         plan = gmms.SegPlan(gmms.INT32, offs)
-        assert plan.segment(0) == (4, 2)
-        assert plan.segment(1) == (64, 2)
-        assert plan.segment(2) == (8, 4)
+        # long segments start from 4096-element runs
+        assert plan.segment(0) == (4, 1)
+        assert plan.segment(1) == (64, 1)
+        assert plan.segment(2) == (8, 3)
         a = oracle.gen_i32(4242, int(offs[-1]), 32)
         got = gmms.segmented_sort(_to_dev(a, dev), offs).cpu().numpy()
         _check_segments(oracle, got, a, offs)

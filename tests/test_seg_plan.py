@@ -10,8 +10,13 @@ def _offsets(lens, start=0):
     return np.concatenate([[start], start + np.cumsum(np.asarray(lens, np.int64))]).astype(np.int64)
 
 
+def _run(n):
+    # segments longer than one 1024-element tile start from 4096-element runs
+    return 4096 if n > 1024 else 1024
+
+
 def _rounds(n, K):
-    r, L = 0, 1024
+    r, L = 0, _run(n)
     while L < n:
         r, L = r + 1, L * K
     return r
@@ -32,13 +37,14 @@ def test_plan_matches_schedule_closed_forms(gmms, K):
     assert info.rounds == max(_rounds(n, K) for n in LENS)
     tasks = 0
     for n in LENS:
-        L = 1024
+        L = _run(n)
         for _ in range(_rounds(n, K)):
             tasks += -(-n // (K * L))
             L *= K
     assert info.merge_items == tasks
     assert info.partition_items == 0
-    assert info.launches == 2 + info.rounds
+    # setup, one base-case launch per run length present, one merge per round
+    assert info.launches == 1 + 2 + info.rounds
     for s, n in enumerate(LENS):
         assert plan.segment(s) == (K, _rounds(n, K))
 
@@ -50,7 +56,7 @@ def test_split_tasks_get_partition_items(gmms):
     # round volumes are 2^22+5000, then 2^22 twice: each is split over ~512 mergers
     assert info.merge_items >= 3 * 400
     assert info.partition_items == info.merge_items - sum(
-        -(-n // (16 * 1024 * 16 ** r)) for n, rr in ((1 << 22, 3), (5000, 1)) for r in range(rr))
+        -(-n // (16 * 4096 * 16 ** r)) for n in (1 << 22, 5000) for r in range(_rounds(n, 16)))
     assert info.launches == 2 + 2 * info.rounds
 
 
@@ -89,8 +95,8 @@ def test_tuning_table_roundtrip_and_validation(gmms):
         gmms.seg_tuning_set(gmms.FLOAT32, kk, mm)
         assert gmms.seg_tuning_get(gmms.FLOAT32) == (kk, mm)
         plan = gmms.SegPlan(gmms.FLOAT32, _offsets([10_000, 20_000]))
-        assert plan.segment(0) == (8, 2)      # class 14: 8193..16384
-        assert plan.segment(1) == (16, 2)     # class 15: default K
+        assert plan.segment(0) == (8, 1)      # class 14: 8193..16384
+        assert plan.segment(1) == (16, 1)     # class 15: default K
         for bad in (3, 128, -2):
             kb = list(kk)
             kb[5] = bad
