@@ -67,6 +67,8 @@ def parse():
                     help="all 8192 segments (about 4e9 keys) instead of the 2^28-key prefix")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--configs", action="store_true",
+                    help="also time the other plain-sort configurations (dtypes, pad and duplicate cases)")
     ap.add_argument("--serve-requests", type=int, default=160)
     ap.add_argument("--serve-rate", type=float, default=200.0, help="offered requests per second")
     return ap.parse_args()
@@ -336,6 +338,67 @@ def serving(args, oracle, threads):
     }
 
 
+def other_configs(oracle, dev, stream):
+    """The other plain-sort configurations: best of 3 sorts on resident
+    buffers, every output checked against the CPU sort."""
+    import numpy as np
+    import torch
+
+    import gmms
+
+    ws = gmms.Workspace()
+
+    def best(keys, vals=None, dt=None):
+        k0 = keys
+        ka = torch.empty_like(keys)
+        va = torch.empty_like(vals) if vals is not None else None
+        ts, res = [], None
+        for it in range(4):
+            k = k0.clone()
+            v = vals.clone() if vals is not None else None
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            alt = gmms.sort_inplace(k, ka, v, va, dtype=dt, workspace=ws, stream=stream)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+            res = ((ka if alt else k), ((va if alt else v) if vals is not None else None))
+        return min(ts[1:]), res
+
+    out = []
+
+    def record(name, n, t, ok):
+        out.append({"workload": name, "n": n, "ms": round(t * 1e3, 3), "gkeys_s": round(n / t / 1e9, 3),
+                    "bit_exact_vs_std_sort": bool(ok)})
+        log(f"configs: {name}: {t * 1e3:.3f} ms ok={ok}")
+
+    n = 10**8 + 17
+    a = oracle.gen_i32(0x6D6D7302, n, 34)
+    t, (r, _) = best(torch.from_numpy(a).to(dev))
+    record("int32 N=10^8+17 uniform in [0, 2^30)", n, t, np.array_equal(r.cpu().numpy(), oracle.sort(a)))
+    n = 1 << 28
+    a = (oracle.gen_u64(0x6D6D7302, n) >> np.uint64(54)).astype(np.int32)
+    t, (r, _) = best(torch.from_numpy(a).to(dev))
+    record("int32 N=2^28 uniform in [0, 1024)", n, t, np.array_equal(r.cpu().numpy(), np.sort(a)))
+    del a
+    x = oracle.gen_f32_normal(0x6D6D7304, n)
+    t, (r, _) = best(torch.from_numpy(x).to(dev))
+    record("float32 N=2^28 normal(0,1), total order", n, t,
+           np.array_equal(r.cpu().numpy().view(np.uint32), oracle.sort(x).view(np.uint32)))
+    del x
+    n = 1 << 27
+    for dist, name in enumerate(("uniform", "sorted", "reverse sorted", "Zipf(1.1) over 2^20")):
+        k, v = oracle.gen_pairs(0x6D6D7310 + dist, n, dist, 1.1, 1 << 20)
+        t, (rk, rv) = best(torch.from_numpy(k.view(np.int64)).to(dev), torch.from_numpy(v.view(np.int32)).to(dev),
+                           gmms.PAIR_U64_U32)
+        wk, wv = oracle.sort_pairs(k, v)
+        ok = np.array_equal(rk.cpu().numpy().view(np.uint64), wk) and np.array_equal(rv.cpu().numpy().view(np.uint32), wv)
+        record(f"(uint64 key, uint32 value=index) N=2^27, keys {name}", n, t, ok)
+    u = oracle.gen_u64(77, n)
+    t, (r, _) = best(torch.from_numpy(u.view(np.int64)).to(dev), dt=gmms.UINT64)
+    record("uint64 N=2^27 uniform", n, t, np.array_equal(r.cpu().numpy().view(np.uint64), oracle.sort(u)))
+    return out
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -461,6 +524,7 @@ def main():
     if not args.no_batched:
         batch = batched(args, oracle, dev, stream, threads)
 
+    extra = other_configs(oracle, dev, stream) if args.configs else None
     serve = None
     if not args.no_serving:
         serve = serving(args, oracle, threads)
@@ -502,6 +566,7 @@ def main():
         "merge_round_ms": per_round,
         "batched": batch,
         "serving": serve,
+        "configs": extra,
     }
     print(json.dumps(out), flush=True)
     if not bit_exact or (batch is not None and not (batch["bit_exact_vs_std_sort"] and batch["identical_to_per_array"])):
