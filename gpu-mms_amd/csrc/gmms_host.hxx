@@ -93,9 +93,10 @@ inline int resolve(int dtype, const gmms_params* p, Resolved* r, bool need_devic
   };
   int K = q.kway;
   if (K == 0) {
-    // wavefront merger: 16 lists (64- or 32-element windows); the others:
-    // the largest power of two that fits LDS
-    K = kern == GMMS_MERGE_WAVE ? 16 : MAX_K;
+    // wavefront merger and heap: 16 lists (the heap: more resident mergers
+    // than with 64); the workgroup tile merger: the largest power of two
+    // that fits LDS
+    K = kern == GMMS_MERGE_TILE ? MAX_K : 16;
     while (K > 2 && lds_bytes(K) > LDS_LIMIT) K >>= 1;
   }
   if (K < 2 || K > MAX_K || (K & (K - 1))) return GMMS_ERROR_INVALID_VALUE;
@@ -105,9 +106,17 @@ inline int resolve(int dtype, const gmms_params* p, Resolved* r, bool need_devic
   r->K = K;
   r->PL = ilog2(K);
   r->D = D;
-  // resident mergers per CU: heap 2 (LDS), workgroup tile 3, wavefront tile
-  // as many workgroups of WM_WAVES mergers as fit the LDS
-  const int per_cu = kern == GMMS_MERGE_HEAP ? 2 : (kern == GMMS_MERGE_TILE ? 3 : wm_resident_per_cu(W));
+  // resident mergers per CU: as many as the LDS holds (heap: one workgroup
+  // of log2(K) wavefronts per merger, at most 32 wavefronts per CU; workgroup
+  // tile 3; wavefront tile: workgroups of WM_WAVES mergers)
+  int per_cu;
+  if (kern == GMMS_MERGE_HEAP) {
+    const int by_lds = (int)std::max<size_t>(1, LDS_LIMIT / lds_bytes(K));
+    const int by_waves = std::max(1, 32 / std::max(1, ilog2(K)));
+    per_cu = std::min(by_lds, by_waves);
+  } else {
+    per_cu = kern == GMMS_MERGE_TILE ? 3 : wm_resident_per_cu(W);
+  }
   r->mergers = q.mergers ? q.mergers : per_cu * (need_device ? cu_count() : 256);
   r->slack = q.slack;
   if (q.base_run != 0 && q.base_run != TILE && q.base_run != TILE * BASE_WAVES) return GMMS_ERROR_INVALID_VALUE;

@@ -299,32 +299,20 @@ __device__ __forceinline__ void wait_vm_ge(int n) {
 }
 __device__ __forceinline__ void wait_vm(int n) { wait_vm_ge<31>(n); }
 
+template <int D, class E>
+__device__ __forceinline__ void clean64_step(E& v, int lane) {
+  const bool low = (lane & D) == 0;
+  v = lane_ce(v, xlane<D>(v), low, low ? 0u : 0xFFFFFFFFu);
+}
+
 template <class E>
 __device__ __forceinline__ void bitonic_clean64(E& v, int lane) {
-  {
-    E p = xlane<32>(v);
-    v = (lane & 32) ? emax(v, p) : emin(v, p);
-  }
-  {
-    E p = xlane<16>(v);
-    v = (lane & 16) ? emax(v, p) : emin(v, p);
-  }
-  {
-    E p = xlane<8>(v);
-    v = (lane & 8) ? emax(v, p) : emin(v, p);
-  }
-  {
-    E p = xlane<4>(v);
-    v = (lane & 4) ? emax(v, p) : emin(v, p);
-  }
-  {
-    E p = xlane<2>(v);
-    v = (lane & 2) ? emax(v, p) : emin(v, p);
-  }
-  {
-    E p = xlane<1>(v);
-    v = (lane & 1) ? emax(v, p) : emin(v, p);
-  }
+  clean64_step<32>(v, lane);
+  clean64_step<16>(v, lane);
+  clean64_step<8>(v, lane);
+  clean64_step<4>(v, lane);
+  clean64_step<2>(v, lane);
+  clean64_step<1>(v, lane);
 }
 
 template <class E>

@@ -62,8 +62,8 @@ enum {
 };
 typedef struct gmms_params {
   int32_t kway;      /* K: lists merged per round; 0: auto (16 for the
-                        wavefront merger; otherwise the largest power of two
-                        <= 64 whose LDS state fits)                          */
+                        wavefront and heap mergers; for the workgroup tile
+                        merger the largest power of two <= 64 that fits LDS) */
   int32_t pipeline;  /* merge kernel, GMMS_MERGE_*                           */
   int32_t mergers;   /* target mergers per round when tasks are few; 0: auto */
   int32_t prefetch;  /* blocks prefetched per list into LDS (1..8)           */
