@@ -349,13 +349,17 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
       // register tile: lane l holds slots l*NREG .. l*NREG+NREG-1, all in run r
       const int r = (lane * NREG) / F;
       const int i0 = (lane * NREG) & (F - 1);
-      const int tk = __shfl(sk, r, WAVE);
+      const int wr = __shfl(want, r, WAVE);
       const int hd = __shfl(head, r, WAVE);
       E v[NREG];
 #pragma unroll
-      for (int q = 0; q < NREG; ++q) {
-        const E x = lget<E>(win, WIN, r * F + ((hd + i0 + q) & (F - 1)));
-        v[q] = sel(i0 + q < tk, x, Limits<E>::hi());
+      for (int q = 0; q < NREG; ++q) v[q] = lget<E>(win, WIN, r * F + ((hd + i0 + q) & (F - 1)));
+      // The untaken elements of a window are > y, so they sort behind the S
+      // taken ones and may stay; only slots past a list's window contents
+      // (a list near its end) need the filler.
+      if (__ballot(i0 + NREG > wr) != 0) {
+#pragma unroll
+        for (int q = 0; q < NREG; ++q) v[q] = sel(i0 + q < wr, v[q], Limits<E>::hi());
       }
       __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): window reads done before the DMA reuses the slots
       refill(cur2, head2, cnt2, want2);
@@ -388,10 +392,11 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
       }
     }
     wait_dma();
+    const int nu = (S + WAVE - 1) / WAVE;  // store rows, the last one partial
 #pragma unroll
     for (int u = 0; u < NREG; ++u) {
       const int o = u * WAVE + lane;
-      if (o < S) gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + o, ov[u]);
+      if (u < nu - 1 || (u == nu - 1 && o < S)) gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + o, ov[u]);
     }
 
     // ---- advance
