@@ -22,7 +22,8 @@ using namespace gmms_host;
 namespace {
 
 template <int DT>
-int launch_basecase(void* keys, uint32_t* vals, int64_t n, bool cin, bool cout, bool run4, hipStream_t s) {
+int launch_basecase(void* keys, uint32_t* vals, int64_t n, bool cin, bool cout, bool run4, uint32_t flip,
+                    hipStream_t s) {
   if (n <= 0) return GMMS_SUCCESS;
   StageScope scope(GMMS_STAGE_BASECASE, n, s);
   const int64_t tiles = (n + TILE - 1) / TILE;
@@ -30,16 +31,16 @@ int launch_basecase(void* keys, uint32_t* vals, int64_t n, bool cin, bool cout, 
   const dim3 block(BASE_WAVES * WAVE);
   if (run4) {
     // the sort's own base case: maps the caller's bits in, and out only if no round follows
-    if (cout) hipLaunchKernelGGL((k_basecase<DT, true, true, true>), grid, block, 0, s, keys, vals, n);
-    else hipLaunchKernelGGL((k_basecase<DT, true, false, true>), grid, block, 0, s, keys, vals, n);
+    if (cout) hipLaunchKernelGGL((k_basecase<DT, true, true, true>), grid, block, 0, s, keys, vals, n, flip);
+    else hipLaunchKernelGGL((k_basecase<DT, true, false, true>), grid, block, 0, s, keys, vals, n, flip);
   } else if (cin && cout) {
-    hipLaunchKernelGGL((k_basecase<DT, true, true>), grid, block, 0, s, keys, vals, n);
+    hipLaunchKernelGGL((k_basecase<DT, true, true>), grid, block, 0, s, keys, vals, n, flip);
   } else if (cin) {
-    hipLaunchKernelGGL((k_basecase<DT, true, false>), grid, block, 0, s, keys, vals, n);
+    hipLaunchKernelGGL((k_basecase<DT, true, false>), grid, block, 0, s, keys, vals, n, flip);
   } else if (cout) {
-    hipLaunchKernelGGL((k_basecase<DT, false, true>), grid, block, 0, s, keys, vals, n);
+    hipLaunchKernelGGL((k_basecase<DT, false, true>), grid, block, 0, s, keys, vals, n, flip);
   } else {
-    hipLaunchKernelGGL((k_basecase<DT, false, false>), grid, block, 0, s, keys, vals, n);
+    hipLaunchKernelGGL((k_basecase<DT, false, false>), grid, block, 0, s, keys, vals, n, flip);
   }
   HIP_OK(hipGetLastError());
   return GMMS_SUCCESS;
@@ -157,6 +158,7 @@ RoundArgs round_args(const void* ik, const void* iv, void* ok, void* ov, int64_t
   a.cuts = (int64_t*)ws;
   a.skips = ws ? (int64_t*)ws + b * r.K : nullptr;
   a.cut_stride = r.K;
+  a.flip = r.flip;
   return a;
 }
 
@@ -181,7 +183,7 @@ int run_sort(void* k0, void* k1, void* v0, void* v1, int64_t n, const Resolved& 
              hipStream_t s, int* alt) {
   int rounds = 0;
   for (int64_t L = r.run; L < n; L *= r.K) ++rounds;
-  int st = launch_basecase<DT>(k0, (uint32_t*)v0, n, true, rounds == 0, r.run > TILE, s);
+  int st = launch_basecase<DT>(k0, (uint32_t*)v0, n, true, rounds == 0, r.run > TILE, r.flip, s);
   if (st) return st;
   void* kb[2] = {k0, k1};
   void* vb[2] = {v0, v1};
@@ -291,6 +293,7 @@ int gmms_resolve_params(int dtype, const gmms_params* in, gmms_params* out) {
   out->prefetch = r.D;
   out->slack = (int32_t)r.slack;
   out->base_run = r.run;
+  out->descending = r.flip ? 1 : 0;
   return GMMS_SUCCESS;
 }
 
@@ -408,7 +411,7 @@ int gmms_squaresort(int dtype, void* keys, void* vals, int64_t n, int to_interna
   hipStream_t s = (hipStream_t)stream;
   return dispatch(dtype, [&](auto dt) {
     return launch_basecase<decltype(dt)::value>(keys, (uint32_t*)vals, n, to_internal != 0, to_external != 0, false,
-                                                s);
+                                                0u, s);
   });
 }
 

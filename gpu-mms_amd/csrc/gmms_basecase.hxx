@@ -150,7 +150,8 @@ __device__ __forceinline__ void wave_exchange(E (&v)[TPL], uint32_t* buf, int wa
 // flip and distance-1024 stages go through LDS between wavefronts, the rest
 // are the in-wavefront stages of merge_level).
 template <int DT, bool CONV_IN, bool CONV_OUT, bool RUN4 = false>
-__global__ __launch_bounds__(BASE_WAVES * WAVE) void k_basecase(void* keys, uint32_t* vals, int64_t n) {
+__global__ __launch_bounds__(BASE_WAVES * WAVE) void k_basecase(void* keys, uint32_t* vals, int64_t n,
+                                                               uint32_t flip) {
   using E = typename DType<DT>::E;
   __shared__ uint32_t xp[BASE_WAVES * E::W * XPOSE_WORDS];
   const int lane = threadIdx.x & (WAVE - 1);
@@ -167,7 +168,7 @@ __global__ __launch_bounds__(BASE_WAVES * WAVE) void k_basecase(void* keys, uint
 #pragma unroll
   for (int r = 0; r < TPL; ++r) {
     const int idx = r * WAVE + lane;
-    v[r] = idx < cnt ? gload<DT, CONV_IN>(keys, vals, base + idx) : Limits<E>::hi();
+    v[r] = idx < cnt ? gload<DT, CONV_IN>(keys, vals, base + idx, flip) : Limits<E>::hi();
   }
 
   sort_tile(v, lane);
@@ -204,7 +205,7 @@ __global__ __launch_bounds__(BASE_WAVES * WAVE) void k_basecase(void* keys, uint
     uint32_t w[E::W];
 #pragma unroll
     for (int q = 0; q < E::W; ++q) w[q] = my[q * XPOSE_WORDS + slot];
-    if (e < cnt) gstore<DT, CONV_OUT>(keys, vals, base + e, from_words<E>(w));
+    if (e < cnt) gstore<DT, CONV_OUT>(keys, vals, base + e, from_words<E>(w), flip);
   }
 }
 
@@ -224,7 +225,8 @@ struct TileDesc {
 // wavefront per descriptor of up to 1024 elements.
 template <int DT, bool RUN4 = false>
 __global__ __launch_bounds__(BASE_WAVES* WAVE) void k_basecase_seg(void* k0, uint32_t* v0, void* k1, uint32_t* v1,
-                                                                    const TileDesc* tiles, int64_t n_tiles) {
+                                                                    const TileDesc* tiles, int64_t n_tiles,
+                                                                    uint32_t flip) {
   using E = typename DType<DT>::E;
   __shared__ uint32_t xp[BASE_WAVES * E::W * XPOSE_WORDS];
   const int lane = threadIdx.x & (WAVE - 1);
@@ -240,7 +242,7 @@ __global__ __launch_bounds__(BASE_WAVES* WAVE) void k_basecase_seg(void* k0, uin
 #pragma unroll
   for (int r = 0; r < TPL; ++r) {
     const int idx = r * WAVE + lane;
-    v[r] = idx < cnt ? gload<DT, true>(k0, v0, base + idx) : Limits<E>::hi();
+    v[r] = idx < cnt ? gload<DT, true>(k0, v0, base + idx, flip) : Limits<E>::hi();
   }
 
   sort_tile(v, lane);
@@ -276,7 +278,7 @@ __global__ __launch_bounds__(BASE_WAVES* WAVE) void k_basecase_seg(void* k0, uin
 #pragma unroll
     for (int q = 0; q < E::W; ++q) w[q] = my[q * XPOSE_WORDS + slot];
     if (e < cnt) {
-      if (conv) gstore<DT, true>(dk, dv, base + e, from_words<E>(w));
+      if (conv) gstore<DT, true>(dk, dv, base + e, from_words<E>(w), flip);
       else gstore<DT, false>(dk, dv, base + e, from_words<E>(w));
     }
   }

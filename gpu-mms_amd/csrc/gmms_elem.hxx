@@ -200,31 +200,37 @@ template <> struct DType<DT_PAIR> {
 };
 
 // Global element access. keys: Word array; vals: u32 array (pairs only).
+// Global load/store of one element; CONV maps the caller's bits to the
+// internal order and back. flip (0 or all ones) XORs every internal word on
+// that mapping: all ones reverses the order (descending sorts), so the
+// kernels in between never know.
 template <int DT, bool CONV>
-__device__ __forceinline__ typename DType<DT>::E gload(const void* keys, const uint32_t* vals, int64_t i) {
+__device__ __forceinline__ typename DType<DT>::E gload(const void* keys, const uint32_t* vals, int64_t i,
+                                                      uint32_t flip = 0u) {
   using D = DType<DT>;
   using E = typename D::E;
   using Wd = typename D::Word;
   Wd x = static_cast<const Wd*>(keys)[i];
-  if (CONV) x = D::in(x);
+  if (CONV) x = D::in(x) ^ (Wd)(int32_t)flip;  // sign-extends to all 64 bits
   E e;
   if constexpr (E::HAS_VAL) {
     e.k = x;
-    e.v = vals[i];
+    e.v = CONV ? vals[i] ^ flip : vals[i];
   } else {
     e.k = x;
   }
   return e;
 }
 template <int DT, bool CONV>
-__device__ __forceinline__ void gstore(void* keys, uint32_t* vals, int64_t i, const typename DType<DT>::E& e) {
+__device__ __forceinline__ void gstore(void* keys, uint32_t* vals, int64_t i, const typename DType<DT>::E& e,
+                                       uint32_t flip = 0u) {
   using D = DType<DT>;
   using E = typename D::E;
   using Wd = typename D::Word;
   Wd x = (Wd)e.k;
-  if (CONV) x = D::out(x);
+  if (CONV) x = D::out(x ^ (Wd)(int32_t)flip);
   static_cast<Wd*>(keys)[i] = x;
-  if constexpr (E::HAS_VAL) vals[i] = e.v;
+  if constexpr (E::HAS_VAL) vals[i] = CONV ? e.v ^ flip : e.v;
 }
 
 }  // namespace gmms

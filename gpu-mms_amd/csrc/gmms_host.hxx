@@ -26,6 +26,7 @@ struct Resolved {
   int64_t slack;
   int kernel;  // GMMS_MERGE_* : wavefront tile, pipelined heap, workgroup tile
   int run;     // sorted run length after the base case (1024 or 4096)
+  uint32_t flip;  // 0, or all ones for a descending sort
 };
 
 enum { GMMS_MERGE_WAVE = 0, GMMS_MERGE_HEAP = 1, GMMS_MERGE_TILE = 2 };
@@ -121,6 +122,8 @@ inline int resolve(int dtype, const gmms_params* p, Resolved* r, bool need_devic
   r->slack = q.slack;
   if (q.base_run != 0 && q.base_run != TILE && q.base_run != TILE * BASE_WAVES) return GMMS_ERROR_INVALID_VALUE;
   r->run = q.base_run ? q.base_run : TILE * BASE_WAVES;
+  if (q.descending != 0 && q.descending != 1) return GMMS_ERROR_INVALID_VALUE;
+  r->flip = q.descending ? 0xFFFFFFFFu : 0u;
   return GMMS_SUCCESS;
 }
 

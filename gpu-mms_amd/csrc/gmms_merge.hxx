@@ -65,6 +65,7 @@ struct RoundArgs {
   int64_t* cuts;  // [(task*m + j) * cut_stride + k]
   int64_t* skips; // [task*m + j]
   int cut_stride; // entries per cut row: K (plain sort) or MAX_K (batched)
+  uint32_t flip;  // XOR of the internal bits on the final store (descending)
   // batched segments: merger descriptors instead of the uniform task grid
   const struct MergerInfo* items;  // nullptr for a plain sort
   int64_t n_items;
@@ -424,7 +425,7 @@ __global__ __launch_bounds__(MAX_PL * WAVE) void k_merge(RoundArgs a) {
   for (int64_t it = 0; it < iters; ++it) {
     if (wave == 0) {
       const int64_t o = it * WAVE + lane - pre;
-      if (o >= 0 && o < count) gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + o, root);
+      if (o >= 0 && o < count) gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + o, root, a.flip);
     }
     if (it == iters - 1) break;
     // root-to-leaf path: the child with the smaller maximum is emptied

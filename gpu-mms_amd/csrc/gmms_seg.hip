@@ -95,6 +95,7 @@ struct gmms_seg_plan {
   int W = 1;
   int R = 0;
   int slots = 0;  // resident mergers (for the persistent grid)
+  uint32_t flip = 0;  // descending: XOR of the internal bits
   int64_t n_seg = 0, elements = 0, extent = 0, tiles = 0;
   int64_t runs4 = 0, runs1 = 0;  // run descriptors: 4096-element (first), 1024-element
   std::vector<int32_t> seg_K, seg_R;  // per segment (all, empties included)
@@ -137,6 +138,7 @@ int plan_build(gmms_seg_plan* P, int dtype, const int64_t* offsets, int64_t n_se
   P->W = W;
   P->n_seg = n_seg;
   P->slots = (p && p->mergers > 0) ? p->mergers : wm_resident_per_cu(W) * cu_count();
+  P->flip = r.flip;
   const int kmax = max_seg_k(W);
   const bool forced = p && p->kway > 0;
   Tuning tune;
@@ -343,13 +345,13 @@ int seg_enqueue(const gmms_seg_plan* P, void* k0, void* k1, void* v0, void* v1, 
     std::unique_ptr<StageScope> sc(timed ? new StageScope(GMMS_STAGE_BASECASE, P->elements, s) : nullptr);
     if (P->runs4 > 0) {
       hipLaunchKernelGGL((k_basecase_seg<DT, true>), dim3((unsigned)P->runs4), dim3(BASE_WAVES * WAVE), 0, s, k0,
-                         (uint32_t*)v0, k1, (uint32_t*)v1, (const TileDesc*)tiles, P->runs4);
+                         (uint32_t*)v0, k1, (uint32_t*)v1, (const TileDesc*)tiles, P->runs4, P->flip);
       HIP_OK(hipGetLastError());
     }
     if (P->runs1 > 0) {
       const int64_t grid = (P->runs1 + BASE_WAVES - 1) / BASE_WAVES;
       hipLaunchKernelGGL((k_basecase_seg<DT, false>), dim3((unsigned)grid), dim3(BASE_WAVES * WAVE), 0, s, k0,
-                         (uint32_t*)v0, k1, (uint32_t*)v1, (const TileDesc*)tiles + P->runs4, P->runs1);
+                         (uint32_t*)v0, k1, (uint32_t*)v1, (const TileDesc*)tiles + P->runs4, P->runs1, P->flip);
       HIP_OK(hipGetLastError());
     }
   }
@@ -367,6 +369,7 @@ int seg_enqueue(const gmms_seg_plan* P, void* k0, void* k1, void* v0, void* v1, 
     a.cuts = cuts;
     a.skips = nullptr;
     a.cut_stride = MAX_K;
+    a.flip = P->flip;
     a.slack = 0;  // per task
     a.queue = queue + g;
     if (sr.n_part > 0) {

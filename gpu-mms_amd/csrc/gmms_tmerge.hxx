@@ -269,7 +269,7 @@ __global__ __launch_bounds__(TM_THREADS) void k_tmerge(RoundArgs a) {
       const int64_t dst = *obase_p;
       for (int64_t i = tid; i < len; i += TM_THREADS) {
         const E v = gload<DT, false>(a.in_keys, a.in_vals, src + i);
-        gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, dst + i, v);
+        gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, dst + i, v, a.flip);
       }
       __syncthreads();
       if (tid == 0) {
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(TM_THREADS) void k_tmerge(RoundArgs a) {
 
     // ---- store the merged step
     const int64_t ob = *obase_p;
-    for (int o = tid; o < S; o += TM_THREADS) gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, ob + o, pl_get<E>(src, cap, o));
+    for (int o = tid; o < S; o += TM_THREADS) gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, ob + o, pl_get<E>(src, cap, o), a.flip);
     __syncthreads();
     if (wave == 0) {
       if (lane < K) {

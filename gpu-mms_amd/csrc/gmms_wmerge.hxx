@@ -311,10 +311,11 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
 #pragma unroll
         for (int t = 0; t < 4; ++t) g[t] = gload<DT, false>(a.in_keys, a.in_vals, src + i + t * WAVE);
 #pragma unroll
-        for (int t = 0; t < 4; ++t) gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + i + t * WAVE, g[t]);
+        for (int t = 0; t < 4; ++t) gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + i + t * WAVE, g[t], a.flip);
       }
       for (; i < glen; i += WAVE)
-        gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + i, gload<DT, false>(a.in_keys, a.in_vals, src + i));
+        gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + i, gload<DT, false>(a.in_keys, a.in_vals, src + i),
+                             a.flip);
       // list ky restarts with an empty window; the others keep theirs
       int c2 = want;
       if (lane == ky) {
@@ -396,7 +397,8 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
 #pragma unroll
     for (int u = 0; u < NREG; ++u) {
       const int o = u * WAVE + lane;
-      if (u < nu - 1 || (u == nu - 1 && o < S)) gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + o, ov[u]);
+      if (u < nu - 1 || (u == nu - 1 && o < S))
+        gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + o, ov[u], a.flip);
     }
 
     // ---- advance

@@ -66,7 +66,8 @@ class Params(ctypes.Structure):
         ("prefetch", ctypes.c_int32),
         ("slack", ctypes.c_int32),
         ("base_run", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("descending", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 1),
     ]
 
 
@@ -219,13 +220,15 @@ def default_params() -> Params:
 
 
 def make_params(K: int = 0, mergers: int = 0, prefetch: int = 0, slack: int = 0,
-                pipeline: Optional[int] = None, base_run: int = 0) -> Params:
+                pipeline: Optional[int] = None, base_run: int = 0, descending: bool = False) -> Params:
     """pipeline=1: pipelined minBlockHeap merge; pipeline=0: tile merge.
     base_run: 1024 (the reference's tile) or 4096 sorted elements per base
     run; 0: auto."""
     p = default_params()
     if base_run:
         p.base_run = base_run
+    if descending:
+        p.descending = 1
     if pipeline is not None:
         p.pipeline = int(pipeline)
     if K:

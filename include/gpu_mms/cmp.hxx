@@ -9,12 +9,13 @@
  *   fptr_t    comparator type, int (*)(DATATYPE, DATATYPE)
  *   cmp<T>    "a strictly precedes b": a < b
  *   host_cmp<T>  the same order for host-side checks
+ *   greater<T>, host_greater<T>  the reversed order, a > b (descending)
  *
  * The GPU never calls these functions: libgmms.so runs the built-in order of
  * the element type (integers by '<', floats by IEEE total order, pairs by
- * (key, value)), which is what cmp<T> states. multimergesort.hxx only accepts
- * cmp<DATATYPE> as its comparator argument and rejects any other at compile
- * time, so a caller can never get an order it did not ask for.
+ * (key, value)), or its exact reverse. multimergesort.hxx accepts cmp<DATATYPE>
+ * and greater<DATATYPE> as its comparator argument and rejects any other at
+ * compile time, so a caller can never get an order it did not ask for.
  * Unlike the reference, MAXVAL is not a sentinel: keys equal to MAXVAL sort
  * like any other key.
  */
@@ -42,6 +43,16 @@ inline int cmp(T a, T b) {
 template <typename T>
 inline int host_cmp(T a, T b) {
   return a < b;
+}
+
+template <typename T>
+inline int greater(T a, T b) {
+  return b < a;
+}
+
+template <typename T>
+inline int host_greater(T a, T b) {
+  return b < a;
 }
 
 #endif

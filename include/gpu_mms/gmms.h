@@ -71,7 +71,9 @@ typedef struct gmms_params {
   int32_t base_run;  /* sorted run length the base case leaves: 1024 (one
                         wavefront tile, the reference's M) or 4096 (four tiles
                         merged in a workgroup); 0: auto (4096)               */
-  int32_t reserved[2];
+  int32_t descending; /* 1: reverse order (a custom cmp(a,b) = a > b, as the
+                         reference's cmp.hxx allows); 0: ascending          */
+  int32_t reserved[1];
 } gmms_params;
 
 /* Library version string, e.g. "gpu-mms-amd 0.1.0 gfx950". */

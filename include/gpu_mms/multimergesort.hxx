@@ -40,6 +40,12 @@ template <> struct dtype_of<int64_t> { static constexpr int value = GMMS_INT64; 
 template <> struct dtype_of<uint64_t> { static constexpr int value = GMMS_UINT64; };
 template <> struct dtype_of<double> { static constexpr int value = GMMS_FLOAT64; };
 
+// Which built-in order a comparator names: 0 cmp<DATATYPE> (ascending),
+// 1 greater<DATATYPE> (descending), -1 anything else.
+template <fptr_t F> struct order_of { static constexpr int value = -1; };
+template <> struct order_of<&cmp<DATATYPE>> { static constexpr int value = 0; };
+template <> struct order_of<&greater<DATATYPE>> { static constexpr int value = 1; };
+
 inline int last_status_storage(int set = -1) {
   static thread_local int s = GMMS_SUCCESS;
   if (set >= 0) s = set;
@@ -54,13 +60,15 @@ inline int multimergesort_status() { return gpu_mms::last_status_storage(); }
 template <typename T, fptr_t f>
 T* multimergesort(T* input, T* output, T* h_data, int threadBlocks, int64_t N) {
   static_assert(std::is_same<T, DATATYPE>::value, "T must be DATATYPE (cmp.hxx)");
-  static_assert(f == &cmp<DATATYPE>, "only cmp<DATATYPE> (a < b) runs on libgmms.so");
+  static_assert(gpu_mms::order_of<f>::value >= 0,
+                "libgmms.so runs cmp<DATATYPE> (a < b) or greater<DATATYPE> (a > b)");
   void* result = nullptr;
   gmms_params p;
   gmms_get_default_params(&p);
   p.kway = K;
   p.pipeline = MERGE_KERNEL;
   p.mergers = threadBlocks > 0 ? threadBlocks : BLOCKS;
+  p.descending = gpu_mms::order_of<f>::value;
   const int st = gmms_multimergesort_params(gpu_mms::dtype_of<T>::value, input, output, h_data, &p, N, &result);
   gpu_mms::last_status_storage(st);
   return st == GMMS_SUCCESS ? static_cast<T*>(result) : nullptr;

@@ -64,6 +64,15 @@ def test_base_run_and_sort_rounds(gmms):
     assert gmms.sort_rounds(gmms.INT32, 1 << 28, gmms.make_params(K=16)) == 4
 
 
+def test_descending_knob(gmms):
+    assert gmms.default_params().descending == 0
+    assert gmms.resolve_params(gmms.INT32, gmms.make_params(descending=True)).descending == 1
+    p = gmms.make_params()
+    p.descending = 2
+    with pytest.raises(gmms.GmmsError):
+        gmms.resolve_params(gmms.INT32, p)
+
+
 def test_num_rounds_matches_reference_loop(gmms, oracle):
     for n in (0, 1, 1024, 1025, 1 << 20, 10**8 + 17, 1 << 28, 1 << 33):
         for K in (2, 8, 64):

@@ -241,3 +241,38 @@ def test_base_run_lengths(gmms, oracle, dev, run):
     x = oracle.gen_u64(6, 50_003).view(np.float64).copy()
     got = _host(gmms.sort(torch.from_numpy(x).to(dev), params=p))
     assert np.array_equal(got.view(np.uint64), oracle.sort(x).view(np.uint64))
+
+
+@pytest.mark.parametrize("run", [1024, 4096])
+def test_descending(gmms, oracle, dev, run):
+    # greater<T> (cmp(a, b) = a > b): the exact reverse of the ascending order
+    import torch
+
+    p = gmms.make_params(descending=True, base_run=run)
+    for n in (1, 1000, 4097, 300_001):
+        print(f"descending int32 n={n} base_run={run}", flush=True)
+        a = oracle.gen_i32(n + 17, n, 32)
+        got = _host(gmms.sort(_to_dev(a, dev), params=p))
+        assert np.array_equal(got, oracle.sort(a)[::-1]), n
+    x = oracle.gen_f32_normal(3, 70_001)
+    x.view(np.uint32)[::41] = 0x80000000
+    x.view(np.uint32)[1::41] = 0xFFC00000
+    got = _host(gmms.sort(_to_dev(x, dev), params=p))
+    assert np.array_equal(got.view(np.uint32), oracle.sort(x)[::-1].view(np.uint32))
+    u = oracle.gen_u64(4, 50_001).view(np.int64)  # int64 tensor: signed order
+    got = _host(gmms.sort(torch.from_numpy(u).to(dev), params=p))
+    assert np.array_equal(got, oracle.sort(u)[::-1])
+    k, v = oracle.gen_pairs(9, 60_000, 3, 1.1, 1 << 20)
+    ks, vs = gmms.sort(torch.from_numpy(k.view(np.int64)).to(dev), torch.from_numpy(v.view(np.int32)).to(dev),
+                       params=p)
+    wk, wv = oracle.sort_pairs(k, v)
+    assert np.array_equal(_host(ks).view(np.uint64), wk[::-1]) and np.array_equal(_host(vs).view(np.uint32), wv[::-1])
+
+
+def test_descending_segments(gmms, oracle, dev):
+    offs = np.array([0, 1, 1000, 5000, 70_000, 70_000, 200_003], np.int64)
+    a = oracle.gen_i32(21, int(offs[-1]), 32)
+    got = _host(gmms.segmented_sort(_to_dev(a, dev), offs, params=gmms.make_params(descending=True)))
+    for s in range(len(offs) - 1):
+        x, y = int(offs[s]), int(offs[s + 1])
+        assert np.array_equal(got[x:y], oracle.sort(a[x:y])[::-1]), s
