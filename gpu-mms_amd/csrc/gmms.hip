@@ -47,11 +47,20 @@ int launch_basecase(void* keys, uint32_t* vals, int64_t n, bool cin, bool cout, 
 }
 
 template <int DT>
-int launch_partition(RoundArgs& a, hipStream_t s) {
+int launch_partition(RoundArgs& a, hipStream_t s, void* samp_k = nullptr, uint32_t* samp_v = nullptr) {
   const int64_t waves = a.tasks * (a.m - 1);
   if (waves <= 0) return GMMS_SUCCESS;
   const dim3 grid((unsigned)((waves + PART_WAVES - 1) / PART_WAVES));
   StageScope scope(GMMS_STAGE_PARTITION, a.n, s);
+  if (samp_k) {
+    // gather the sample array of the round's input; the searches start there
+    const int64_t cnt = (a.n + SAMP - 1) >> SAMP_SHIFT;
+    hipLaunchKernelGGL((k_sample<DT>), dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, a.in_keys, a.in_vals,
+                       a.n, samp_k, samp_v);
+    HIP_OK(hipGetLastError());
+    a.samp_keys = samp_k;
+    a.samp_vals = samp_v;
+  }
   hipLaunchKernelGGL((k_partition<DT>), grid, dim3(PART_WAVES * WAVE), 0, s, a);
   HIP_OK(hipGetLastError());
   return GMMS_SUCCESS;
@@ -162,16 +171,39 @@ RoundArgs round_args(const void* ik, const void* iv, void* ok, void* ov, int64_t
   return a;
 }
 
+// One merge round. The partition searches the sample array of the input
+// (every SAMP-th element, at the start of ws) when the lists start at
+// multiples of SAMP: gathered here unless `samp_in` says the previous round
+// wrote it. `samp_out`: the wavefront merger writes the samples of its
+// output for the next round (the return value says whether it did).
 template <int DT>
 int run_round(const void* ik, const void* iv, void* ok, void* ov, int64_t n, int64_t L, const Resolved& r,
-              bool cout, void* ws, size_t wsb, hipStream_t s) {
+              bool cout, void* ws, size_t wsb, hipStream_t s, bool samp_in = false, bool samp_out = false,
+              bool* wrote = nullptr) {
   const int m = mergers_per_task(n, L, r.K, r.mergers);
-  RoundArgs a = round_args(ik, iv, ok, ov, n, L, r, m, ws);
+  const size_t sb = samp_bytes(n, r.W);
+  RoundArgs a = round_args(ik, iv, ok, ov, n, L, r, m, ws ? (char*)ws + sb : nullptr);
+  const size_t cnt = (size_t)((n + SAMP - 1) >> SAMP_SHIFT);
+  uint32_t* sv = (ws && r.W == 3) ? (uint32_t*)((char*)ws + cnt * 8) : nullptr;
   if (m > 1) {
-    const size_t need = (size_t)(a.tasks * m) * (size_t)(r.K + 1) * sizeof(int64_t);
-    if (!ws || wsb < need) return GMMS_ERROR_WORKSPACE;
-    int st = launch_partition<DT>(a, s);
+    if (!ws || wsb < sb + cut_bytes(n, L, r)) return GMMS_ERROR_WORKSPACE;
+    int st;
+    if (L % SAMP != 0) {
+      st = launch_partition<DT>(a, s);
+    } else if (samp_in) {
+      a.samp_keys = ws;
+      a.samp_vals = sv;
+      st = launch_partition<DT>(a, s);
+    } else {
+      st = launch_partition<DT>(a, s, ws, sv);
+    }
     if (st) return st;
+  }
+  if (wrote) *wrote = false;
+  if (samp_out && !cout && r.kernel == GMMS_MERGE_WAVE && ws && wsb >= sb) {
+    a.samp_out_keys = ws;
+    a.samp_out_vals = sv;
+    if (wrote) *wrote = true;
   }
   if (r.kernel == GMMS_MERGE_HEAP) return launch_merge<DT>(a, cout, s);
   if (r.kernel == GMMS_MERGE_TILE) return launch_tmerge<DT>(a, cout, s);
@@ -188,8 +220,14 @@ int run_sort(void* k0, void* k1, void* v0, void* v1, int64_t n, const Resolved& 
   void* kb[2] = {k0, k1};
   void* vb[2] = {v0, v1};
   int cur = 0, done = 0;
+  bool samp = false;  // the sample array of the current input is in ws
   for (int64_t L = r.run; L < n; L *= r.K, ++done) {
-    st = run_round<DT>(kb[cur], vb[cur], kb[cur ^ 1], vb[cur ^ 1], n, L, r, done == rounds - 1, ws, wsb, s);
+    // the next round partitions (and its lists are SAMP-aligned): this
+    // round's merge writes the samples of its output
+    const int64_t L2 = L * r.K;
+    const bool next = L2 < n && L2 % SAMP == 0 && mergers_per_task(n, L2, r.K, r.mergers) > 1;
+    st = run_round<DT>(kb[cur], vb[cur], kb[cur ^ 1], vb[cur ^ 1], n, L, r, done == rounds - 1, ws, wsb, s, samp,
+                       next, &samp);
     if (st) return st;
     cur ^= 1;
   }
@@ -340,10 +378,7 @@ int gmms_round_workspace_size(int dtype, int64_t n, int64_t L, const gmms_params
   if (st) return st;
   *bytes = 0;
   if (n == 0) return GMMS_SUCCESS;
-  const int m = mergers_per_task(n, L, r.K, r.mergers);
-  const int64_t TK = (int64_t)r.K * L;
-  const int64_t tasks = (n + TK - 1) / TK;
-  if (m > 1) *bytes = (size_t)(tasks * m) * (size_t)(r.K + 1) * sizeof(int64_t);
+  if (mergers_per_task(n, L, r.K, r.mergers) > 1) *bytes = samp_bytes(n, r.W) + cut_bytes(n, L, r);
   return GMMS_SUCCESS;
 }
 

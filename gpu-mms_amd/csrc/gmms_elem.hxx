@@ -232,5 +232,31 @@ __device__ __forceinline__ void gstore(void* keys, uint32_t* vals, int64_t i, co
   static_cast<Wd*>(keys)[i] = x;
   if constexpr (E::HAS_VAL) vals[i] = CONV ? e.v ^ flip : e.v;
 }
+// Four consecutive elements e[0..3] to positions base + o .. base + o + 3
+// with 16-byte vector stores (two for 64-bit keys). base is wave-uniform and
+// the addresses need only element alignment.
+template <int DT, bool CONV>
+__device__ __forceinline__ void gstore4(void* keys, uint32_t* vals, int64_t base, int o,
+                                        const typename DType<DT>::E* e, uint32_t flip = 0u) {
+  using D = DType<DT>;
+  using E = typename D::E;
+  using Wd = typename D::Word;
+  typedef Wd Wd4 __attribute__((ext_vector_type(4), aligned(sizeof(Wd))));
+  Wd4 x;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    Wd k = (Wd)e[t].k;
+    if (CONV) k = D::out(k ^ (Wd)(int32_t)flip);
+    x[t] = k;
+  }
+  *(Wd4*)(static_cast<Wd*>(keys) + base + o) = x;
+  if constexpr (E::HAS_VAL) {
+    typedef uint32_t V4 __attribute__((ext_vector_type(4), aligned(4)));
+    V4 y;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) y[t] = CONV ? e[t].v ^ flip : e[t].v;
+    *(V4*)(vals + base + o) = y;
+  }
+}
 
 }  // namespace gmms

@@ -27,22 +27,25 @@ struct Resolved {
   int kernel;  // GMMS_MERGE_* : wavefront tile, pipelined heap, workgroup tile
   int run;     // sorted run length after the base case (1024 or 4096)
   uint32_t flip;  // 0, or all ones for a descending sort
+  int W;          // 32-bit words per element
 };
 
 enum { GMMS_MERGE_WAVE = 0, GMMS_MERGE_HEAP = 1, GMMS_MERGE_TILE = 2 };
 
-// wavefront-merger window budget (elements): 13 KB of LDS per wavefront
+// wavefront-merger window budget (elements): the windows and the transpose
+// take 8 KB of LDS per wavefront for 32- and 64-bit keys, 12 KB for pairs
 constexpr int wm_win(int W) { return W == 1 ? 1024 : 512; }
 
 constexpr int TM_F = 64;  // tile-merge window per list
 
 constexpr size_t LDS_LIMIT = 160 * 1024;  // gfx950 LDS per workgroup
 
-// resident wavefront mergers per CU for W-word elements
+// resident wavefront mergers per CU for W-word elements: what the LDS holds,
+// at most 16 (4 wavefronts per SIMD: the kernel's register budget)
 inline int wm_resident_per_cu(int W) {
   const size_t per_wg = WM_WAVES * wmerge_lds_words_per_wave(W, wm_win(W)) * sizeof(uint32_t);
   const int wgs = (int)std::max<size_t>(1, LDS_LIMIT / per_wg);
-  return std::min(wgs * WM_WAVES, 32);
+  return std::min(wgs * WM_WAVES, 16);
 }
 
 inline int ilog2(int x) {
@@ -105,6 +108,7 @@ inline int resolve(int dtype, const gmms_params* p, Resolved* r, bool need_devic
   r->kernel = kern;
   if (q.mergers < 0 || q.slack < 0) return GMMS_ERROR_INVALID_VALUE;
   r->K = K;
+  r->W = W;
   r->PL = ilog2(K);
   r->D = D;
   // resident mergers per CU: as many as the LDS holds (heap: one workgroup
@@ -147,18 +151,24 @@ inline int64_t slack_for(const Resolved& r, int64_t n, int64_t L, int m) {
   return std::max<int64_t>(WAVE, tsz / m / 32);
 }
 
+// Workspace of a round with more than one merger per task: the sample array
+// of its input (every SAMP-th element: keys, then values for pairs; 256-byte
+// aligned), then the cut rows and skips.
+inline size_t samp_bytes(int64_t n, int W) {
+  const size_t cnt = (size_t)((n + SAMP - 1) >> SAMP_SHIFT);
+  const size_t b = cnt * (W == 1 ? 4 : 8) + (W == 3 ? cnt * 4 : 0);
+  return (b + 255) & ~(size_t)255;
+}
+inline size_t cut_bytes(int64_t n, int64_t L, const Resolved& r) {
+  const int64_t TK = (int64_t)r.K * L;
+  const int64_t tasks = (n + TK - 1) / TK;
+  const int m = mergers_per_task(n, L, r.K, r.mergers);
+  return m > 1 ? (size_t)(tasks * m) * (size_t)(r.K + 1) * sizeof(int64_t) : 0;
+}
 inline size_t ws_bytes_for(int64_t n, const Resolved& r) {
   size_t need = 0;
-  for (int64_t L = r.run; L < n; L *= r.K) {
-    const int64_t TK = (int64_t)r.K * L;
-    const int64_t tasks = (n + TK - 1) / TK;
-    const int m = mergers_per_task(n, L, r.K, r.mergers);
-    if (m > 1) {
-      const size_t b = (size_t)(tasks * m);
-      need = std::max(need, b * (size_t)(r.K + 1) * sizeof(int64_t));
-    }
-  }
-  return need;
+  for (int64_t L = r.run; L < n; L *= r.K) need = std::max(need, cut_bytes(n, L, r));
+  return need ? samp_bytes(n, r.W) + need : 0;
 }
 
 extern thread_local char g_last_error[256];

@@ -70,7 +70,29 @@ struct RoundArgs {
   const struct MergerInfo* items;  // nullptr for a plain sort
   int64_t n_items;
   unsigned int* queue;             // work-queue counter of a persistent launch
+  // every SAMP-th input element (internal bits), or nullptr: the partition
+  // searches these first. Lists must start at multiples of SAMP.
+  const void* samp_keys;
+  const uint32_t* samp_vals;
+  // where a merge writes the samples of its output for the next round's
+  // partition (nullptr: none); the wavefront merger writes them
+  void* samp_out_keys;
+  uint32_t* samp_out_vals;
 };
+
+constexpr int SAMP_SHIFT = 8;
+constexpr int64_t SAMP = (int64_t)1 << SAMP_SHIFT;  // sample stride (elements)
+
+// Sample gather: samp[i] = in[i * SAMP] for i < ceil(n / SAMP).
+template <int DT>
+__global__ __launch_bounds__(256) void k_sample(const void* keys, const uint32_t* vals, int64_t n, void* sk,
+                                                uint32_t* sv) {
+  using Wd = typename DType<DT>::Word;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ((n + SAMP - 1) >> SAMP_SHIFT)) return;
+  static_cast<Wd*>(sk)[i] = static_cast<const Wd*>(keys)[i << SAMP_SHIFT];
+  if constexpr (DType<DT>::E::HAS_VAL) sv[i] = vals[i << SAMP_SHIFT];
+}
 
 // Geometry of one merger: task base (element index of its list 0), task size
 // T, list length L, mergers per task m, its index j, and the cut row of its
@@ -135,6 +157,11 @@ template <int DT>
 __device__ __forceinline__ typename DType<DT>::E ld_int(const RoundArgs& a, int64_t i) {
   return gload<DT, false>(a.in_keys, a.in_vals, i);
 }
+// input element i * SAMP, from the sample array
+template <int DT>
+__device__ __forceinline__ typename DType<DT>::E ld_samp(const RoundArgs& a, int64_t i) {
+  return gload<DT, false>(a.samp_keys, a.samp_vals, i);
+}
 
 __device__ __forceinline__ int wave_sum32(int x) {
 #pragma unroll
@@ -178,29 +205,43 @@ __device__ __forceinline__ void partition_one(const RoundArgs& a, const MergerIn
     const bool interp = hi_moves < 2;
     const int64_t tw = Wt >> 1;  // in [0, Wt)
     int64_t pos = lo;
+    bool from_samp = false;
     if (act) {
       int64_t off = interp ? (int64_t)((double)(r - Rlo) / (double)Wt * (double)w) : (w >> 1);
       if (off < 0) off = 0;
       pos = lo + (off < w ? off : w - 1);
+      if (a.samp_keys) {
+        // the nearest sample position at or above pos's block start inside
+        // [lo, hi): read from the (cache-resident) sample array
+        int64_t ps = pos & ~(SAMP - 1);
+        if (ps < lo) ps += SAMP;
+        if (ps < hi) {
+          pos = ps;
+          from_samp = true;
+        }
+      }
     }
-    const E x = act ? ld_int<DT>(a, lbase + pos) : Limits<E>::hi();
-    // rank of this candidate among active candidates, order (element, lane)
+    const E x = !act ? Limits<E>::hi() : from_samp ? ld_samp<DT>(a, (lbase + pos) >> SAMP_SHIFT) : ld_int<DT>(a, lbase + pos);
+    // rank of this candidate among active candidates, order (element,
+    // lane); the candidates are read with readlane (no LDS round trips)
+    const unsigned long long amask = __ballot(act);
     int rank = 0;
     for (int i = 0; i < K; ++i) {
-      const bool ai = __shfl((int)act, i, WAVE) != 0;
+      if (!((amask >> i) & 1ull)) continue;  // wave-uniform
       uint32_t wi[E::W];
 #pragma unroll
-      for (int q = 0; q < E::W; ++q) wi[q] = (uint32_t)__shfl((int)word(x, q), i, WAVE);
+      for (int q = 0; q < E::W; ++q) wi[q] = (uint32_t)__builtin_amdgcn_readlane((int)word(x, q), i);
       const E xi = from_words<E>(wi);
-      rank += (ai && (lt(xi, x) || (!lt(x, xi) && i < lane))) ? 1 : 0;
+      rank += (lt(xi, x) || (!lt(x, xi) && i < lane)) ? 1 : 0;
     }
     // inclusive weight prefix in sorted order
     int64_t pre = 0;
     for (int i = 0; i < K; ++i) {
-      const int ri = __shfl(rank, i, WAVE);
-      const int64_t wi = __shfl(act ? w : (int64_t)0, i, WAVE);
-      const bool ai = __shfl((int)act, i, WAVE) != 0;
-      if (ai && ri <= rank) pre += wi;
+      if (!((amask >> i) & 1ull)) continue;
+      const int ri = __builtin_amdgcn_readlane(rank, i);
+      const int64_t wi = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)((uint64_t)w >> 32), i) << 32) |
+                                   (uint32_t)__builtin_amdgcn_readlane((int)w, i));
+      if (ri <= rank) pre += wi;
     }
     const bool is_piv = act && (pre - w) <= tw && tw < pre;
     const unsigned long long bal = __ballot(is_piv);
@@ -211,40 +252,68 @@ __device__ __forceinline__ void partition_one(const RoundArgs& a, const MergerIn
     const E y = from_words<E>(yw);
     const int64_t ppos = __shfl(pos, kp, WAVE);
     // count of list elements ordered before y, inside [lo, hi]: every list
-    // is searched by G = 64/K lanes at once, an A-ary search (A = max(G, 2),
-    // probes g = 0..A-2 at l + (g+1)*n/A; powers of two, so shifts only)
+    // is searched by G = 64/K lanes at once: an A-ary search (A = np*G,
+    // np = 1 probe per lane, 2 when G = 1) with probes g = t*G + gs
+    // (g <= A-2) at l + (g+1)*n/A; powers of two, so shifts only. (More
+    // probes per lane measured slower: the searches are bound by memory
+    // transactions, not by round trips.)
+    constexpr int PR = 2;
     const int G = WAVE / K;
-    const int ab = G >= 2 ? __builtin_ctz((unsigned)G) : 1;  // log2(A)
+    const int np = G >= 2 ? 1 : 2;
+    const int ab = __builtin_ctz((unsigned)(G * np));  // log2(A)
     const int A = 1 << ab;
     const int gl = lane / G;  // this lane's list
-    const int gs = lane % G;  // its probe (gs <= A-2 probes)
+    const int gs = lane % G;
     int64_t l = __shfl(lo, gl, WAVE), h = __shfl(hi, gl, WAVE);
     if (gl == kp) l = h = ppos;     // the pivot's own list: exactly its position
     const bool upper = gl < kp;     // equal keys of lower lists come first
     const int64_t gbase = mi.tbase + (int64_t)gl * mi.L;
     const unsigned long long gmask = G == WAVE ? ~0ull : ((1ull << G) - 1ull);
-    for (;;) {
-      const bool open = l < h;
-      if (__ballot(open) == 0) break;
-      const int64_t n = h - l;
-      const bool probe = open && gs <= A - 2;
-      bool before = true;
-      if (probe) {
-        const E v = ld_int<DT>(a, gbase + l + (((int64_t)(gs + 1) * n) >> ab));
-        before = upper ? !lt(y, v) : lt(v, y);
-      }
-      const unsigned long long nb = (__ballot(probe && !before) >> (gl * G)) & gmask;
-      if (open) {
-        if (nb == 0) {
-          l = l + (((int64_t)(A - 1) * n) >> ab) + 1;  // past the last probe
-        } else {
-          const int g0 = __ffsll((long long)nb) - 1;  // first probe not before y
-          const int64_t hp = l + (((int64_t)(g0 + 1) * n) >> ab);
-          l = g0 == 0 ? l : l + (((int64_t)g0 * n) >> ab) + 1;
-          h = hp;
+    // [sl, sh) -> the first index not before y (sh: none); ld(i) loads index i
+    auto search = [&](int64_t& sl, int64_t& sh, auto ld) {
+      for (;;) {
+        const bool open = sl < sh;
+        if (__ballot(open) == 0) break;
+        const int64_t n = sh - sl;
+        E v[PR];
+#pragma unroll
+        for (int t = 0; t < PR; ++t) {
+          const int g = t * G + gs;
+          if (t < np && open && g <= A - 2) v[t] = ld(sl + (((int64_t)(g + 1) * n) >> ab));
+        }
+        unsigned long long nb = 0;  // bit g: probe g is not before y
+#pragma unroll
+        for (int t = 0; t < PR; ++t) {
+          if (t >= np) break;  // wave-uniform
+          const int g = t * G + gs;
+          const bool probe = open && g <= A - 2;
+          const bool before = !probe || (upper ? !lt(y, v[t]) : lt(v[t], y));
+          nb |= ((__ballot(!before) >> (gl * G)) & gmask) << (t * G);
+        }
+        if (open) {
+          if (nb == 0) {
+            sl = sl + (((int64_t)(A - 1) * n) >> ab) + 1;  // past the last probe
+          } else {
+            const int g0 = __ffsll((long long)nb) - 1;  // first probe not before y
+            const int64_t hp = sl + (((int64_t)(g0 + 1) * n) >> ab);
+            sl = g0 == 0 ? sl : sl + (((int64_t)g0 * n) >> ab) + 1;
+            sh = hp;
+          }
         }
       }
+    };
+    if (a.samp_keys) {
+      // narrow [l, h) on the samples first: sample t sits at list position
+      // t * SAMP; find the first sample in [l, h) not before y, then the
+      // answer lies in the block that ends there
+      const int64_t sl = (l + SAMP - 1) >> SAMP_SHIFT, sh = (h + SAMP - 1) >> SAMP_SHIFT;
+      int64_t tl = sl, th = sh;
+      const int64_t sbase = gbase >> SAMP_SHIFT;
+      search(tl, th, [&](int64_t i) { return ld_samp<DT>(a, sbase + i); });
+      if (tl > sl) l = max(l, ((tl - 1) << SAMP_SHIFT) + 1);
+      if (tl < sh) h = min(h, tl << SAMP_SHIFT);
     }
+    search(l, h, [&](int64_t i) { return ld_int<DT>(a, gbase + i); });
     int64_t c = __shfl(l, (lane % K) * G, WAVE);
     if (!inl) c = 0;
     const int64_t R = wave_sum64(c);

@@ -29,9 +29,10 @@ namespace gmms {
 constexpr int WM_WAVES = 4;  // mergers per workgroup
 
 // LDS words per wavefront merger with a window budget of WIN elements: the
-// windows, and the bank-padded transpose of the register tile for the store.
+// windows, and the transpose of the register tile for the store (bank-padded
+// rows for multi-word elements).
 __host__ __device__ inline size_t wmerge_lds_words_per_wave(int W, int WIN) {
-  return (size_t)WIN * W + (size_t)WAVE * (WIN / WAVE + 1) * W;
+  return W == 1 ? (size_t)2 * WIN : (size_t)WIN * W + (size_t)WAVE * (WIN / WAVE + 1) * W;
 }
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -40,6 +41,27 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// a wave-uniform 64-bit value, held in scalar registers
+__device__ __forceinline__ int64_t uniform64(int64_t x) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)x >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Chunk swizzle of the register-tile transpose (XC = 4 chunks of 4 words
+// per lane row of 16 words): a lane's chunk k goes to chunk position
+// k ^ xswz(lane). Lanes 0,2,4,6 (and 1,3,5,7) get 0,1,3,2, and lanes m, m+4,
+// m+8, m+12 get four distinct values, so the 16-byte accesses of 8
+// consecutive lanes (32 banks) or 16 consecutive lanes (64 banks) touch every
+// bank once.
+template <int XC>
+__device__ __forceinline__ int xswz(int l) {
+  static_assert(XC == 4, "rows of 16 words");
+  return ((l >> 1) ^ (l >> 2)) & 3;
 }
 
 template <class E>
@@ -114,16 +136,21 @@ template <int DT>
 __device__ __forceinline__ void dma_elem(const RoundArgs& a, int64_t base, uint32_t off, uint32_t* dst, int plane) {
   using E = typename DType<DT>::E;
   typedef __attribute__((address_space(3))) void* lds_ptr;
+  // byte offsets as 32-bit values, zero-extended: the address is then
+  // scalar base + 32-bit vector offset, with no 64-bit vector arithmetic
   if constexpr (E::W == 1) {
-    const uint32_t* kb = (const uint32_t*)a.in_keys + base;
-    __builtin_amdgcn_global_load_lds((const void*)(kb + off), (lds_ptr)dst, 4, 0, 0);
+    const char* kb = (const char*)((const uint32_t*)a.in_keys + base);
+    __builtin_amdgcn_global_load_lds((const void*)(kb + (uint32_t)(off * 4u)), (lds_ptr)dst, 4, 0, 0);
   } else {
-    const uint64_t* kb = (const uint64_t*)a.in_keys + base;
-    const uint32_t* kw = (const uint32_t*)(kb + off);
+    const char* kb = (const char*)((const uint64_t*)a.in_keys + base);
+    const char* kw = kb + (uint32_t)(off * 8u);
     __builtin_amdgcn_global_load_lds((const void*)kw, (lds_ptr)dst, 4, 0, 0);
-    __builtin_amdgcn_global_load_lds((const void*)(kw + 1), (lds_ptr)(dst + plane), 4, 0, 0);
-    if constexpr (E::W == 3)
-      __builtin_amdgcn_global_load_lds((const void*)(a.in_vals + base + off), (lds_ptr)(dst + 2 * plane), 4, 0, 0);
+    __builtin_amdgcn_global_load_lds((const void*)(kw + 4), (lds_ptr)(dst + plane), 4, 0, 0);
+    if constexpr (E::W == 3) {
+      const char* vb = (const char*)(a.in_vals + base);
+      __builtin_amdgcn_global_load_lds((const void*)(vb + (uint32_t)(off * 4u)), (lds_ptr)(dst + 2 * plane), 4, 0,
+                                       0);
+    }
   }
 }
 
@@ -134,10 +161,17 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
   constexpr int W = E::W;
   constexpr int CHUNKS = WIN / WAVE;
   constexpr int NREG = WIN / WAVE;        // register-tile slots per lane
-  constexpr int XS = NREG + 1;            // padded transpose stride
+  // Store layout. 32-bit keys (VEC): ov[4u + t] is output 256u + 4*lane + t,
+  // stored 16 bytes per lane. Multi-word elements: ov[u] is output
+  // u*64 + lane, stored one element per lane (the 16-byte form needs more
+  // registers than those kernels have to spare).
+  constexpr bool VEC = W == 1;
+  constexpr int XC = NREG / 4;            // VEC: 16-byte chunks per lane = output rows of 256
+  constexpr int XS = VEC ? NREG : NREG + 1;  // transpose row stride (padded for one-word reads)
   constexpr int XW = WAVE * XS;           // transpose words per plane
+  static_assert(!VEC || XC == 4, "the 16-byte transpose covers 16 slots per lane");
   uint32_t* win = my;                     // [list a][F] circular windows, W planes of WIN
-  uint32_t* xp = win + WIN * W;           // transpose of the register tile
+  uint32_t* xp = win + WIN * W;           // transpose of the register tile, W planes of XW
 
   const int K = mi.K;
   const int j = mi.j;
@@ -150,7 +184,8 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
     lo = j == 0 ? 0 : a.cuts[(mi.cut_row + j) * a.cut_stride + lane];
     hi = j == mi.m - 1 ? nk : a.cuts[(mi.cut_row + j + 1) * a.cut_stride + lane];
   }
-  int64_t obase = tbase + wave_sum64(lo);
+  int64_t obase = uniform64(tbase + wave_sum64(lo));
+  const int64_t o0 = obase;  // this merger's output is [o0, final obase)
   // compact the non-empty lists, keeping list order
   const bool act = lane < K && hi > lo;
   const unsigned long long amask = __ballot(act);
@@ -297,7 +332,7 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
         } else {
           glo = ghi;
         }
-        glen = glo;
+        glen = uniform64(glo);
         sk = 0;
       }
     }
@@ -332,18 +367,21 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
     }
 
     // ---- elements taken this step, and the state after it
-    const int S = wave_sum32(sk);
+    const int S = __builtin_amdgcn_readfirstlane(wave_sum32(sk));
     const int64_t cur2 = cur + sk;
     const int head2 = (head + sk) & (F - 1);
     const int cnt2 = want - sk;
     const int want2 = live ? (int)(endp - cur2 < F ? endp - cur2 : F) : 0;
-    // outputs o = u*64 + lane (o < S), in order
+    // outputs (o < S) in the store layout above
     E ov[NREG];
     if (Kp == 1) {
       // one run: it is window 0 (list 0's, held by lane 0), in order from its head
       const int h0 = __builtin_amdgcn_readlane(head, 0);
 #pragma unroll
-      for (int u = 0; u < NREG; ++u) ov[u] = lget<E>(win, WIN, (h0 + u * WAVE + lane) & (F - 1));
+      for (int q = 0; q < NREG; ++q) {
+        const int o = VEC ? (q >> 2) * 256 + 4 * lane + (q & 3) : q * WAVE + lane;
+        ov[q] = lget<E>(win, WIN, (h0 + o) & (F - 1));
+      }
       __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): window reads done before the DMA reuses the slots
       refill(cur2, head2, cnt2, want2);
     } else {
@@ -374,31 +412,80 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
         case 4: merge_level<4>(v, lane); [[fallthrough]];
         default: merge_level<5>(v, lane);
       }
-      // transpose through LDS so the stores are coalesced
+      if constexpr (VEC) {
+        // transpose through LDS in 16-byte chunks so that every lane stores
+        // four consecutive outputs: lane l's slots go to row l (NREG words),
+        // chunk k at chunk position k ^ xswz(l). The swizzle puts the 8 (or
+        // 16) lanes that one LDS cycle serves on distinct banks, for the
+        // writes and for the reads of output rows of 256.
 #pragma unroll
-      for (int w = 0; w < W; ++w) {
+        for (int k = 0; k < XC; ++k) {
+          const u32x4 c = {word(v[4 * k], 0), word(v[4 * k + 1], 0), word(v[4 * k + 2], 0), word(v[4 * k + 3], 0)};
+          *(u32x4*)(xp + lane * NREG + 4 * (k ^ xswz<XC>(lane))) = c;
+        }
+        wave_lds_sync();
+        // output o = 256u + 4*lane + t sits in tile lane L = o / NREG, chunk
+        // (o % NREG) / 4; row u adds 256 words and leaves the swizzle alone
+        const int L = (4 * lane) / NREG;
+        const int rb = L * NREG + 4 * ((lane % XC) ^ xswz<XC>(L));
 #pragma unroll
-        for (int q = 0; q < NREG; ++q) xp[w * XW + lane * XS + q] = word(v[q], w);
-      }
-      wave_lds_sync();
-      // slot of output o = u*64 + lane: lane (o / NREG), register (o % NREG)
-      const int xb = (lane / NREG) * XS + (lane % NREG);
+        for (int u = 0; u < XC; ++u) {
+          const u32x4 c = *(const u32x4*)(xp + u * 256 + rb);
 #pragma unroll
-      for (int u = 0; u < NREG; ++u) {
-        const int slot = xb + u * (WAVE / NREG) * XS;
-        uint32_t wd[W];
+          for (int t = 0; t < 4; ++t) {
+            const uint32_t wd[1] = {c[t]};
+            ov[4 * u + t] = from_words<E>(wd);
+          }
+        }
+      } else {
+        // transpose through LDS (rows padded to NREG + 1 words against bank
+        // conflicts) so that the one-element stores are coalesced
 #pragma unroll
-        for (int w = 0; w < W; ++w) wd[w] = xp[w * XW + slot];
-        ov[u] = from_words<E>(wd);
+        for (int w = 0; w < W; ++w) {
+#pragma unroll
+          for (int q = 0; q < NREG; ++q) xp[w * XW + lane * XS + q] = word(v[q], w);
+        }
+        wave_lds_sync();
+        // slot of output o = u*64 + lane: lane (o / NREG), register (o % NREG)
+        const int xb = (lane / NREG) * XS + (lane % NREG);
+#pragma unroll
+        for (int u = 0; u < NREG; ++u) {
+          const int slot = xb + u * (WAVE / NREG) * XS;
+          uint32_t wd[W];
+#pragma unroll
+          for (int w = 0; w < W; ++w) wd[w] = xp[w * XW + slot];
+          ov[u] = from_words<E>(wd);
+        }
       }
     }
     wait_dma();
-    const int nu = (S + WAVE - 1) / WAVE;  // store rows, the last one partial
+    if constexpr (VEC) {
+      // rows below S/256 are full (a wave-uniform test); in the partial row a
+      // lane stores its four outputs at once, or one by one at the end
+      const int nfull = S >> 8;
 #pragma unroll
-    for (int u = 0; u < NREG; ++u) {
-      const int o = u * WAVE + lane;
-      if (u < nu - 1 || (u == nu - 1 && o < S))
-        gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + o, ov[u], a.flip);
+      for (int u = 0; u < XC; ++u) {
+        const int o = u * 256 + 4 * lane;
+        if (u < nfull) {
+          gstore4<DT, CONV_OUT>(a.out_keys, a.out_vals, obase, o, &ov[4 * u], a.flip);
+        } else if (u == nfull) {
+          if (o + 4 <= S) {
+            gstore4<DT, CONV_OUT>(a.out_keys, a.out_vals, obase, o, &ov[4 * u], a.flip);
+          } else {
+#pragma unroll
+            for (int t = 0; t < 3; ++t)
+              if (o + t < S) gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + o + t, ov[4 * u + t], a.flip);
+          }
+        }
+      }
+    } else {
+      const int nu = (S + WAVE - 1) / WAVE;  // store rows, the last one partial
+#pragma unroll
+      for (int u = 0; u < NREG; ++u) {
+        const int o = u * WAVE + lane;
+        if (u < nu - 1 || (u == nu - 1 && o < S))
+          gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + o, ov[u], a.flip);
+      }
     }
 
     // ---- advance
@@ -406,6 +493,16 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
     cur = cur2;
     head = head2;
     want = want2;
+  }
+  if (a.samp_out_keys) {
+    // the next round's partition samples (every SAMP-th output element) of
+    // this merger's range: read back from the output this wavefront has just
+    // written (its own stores, so they are visible to it in order)
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    const int64_t s1 = (obase + SAMP - 1) >> SAMP_SHIFT;
+    for (int64_t t = ((o0 + SAMP - 1) >> SAMP_SHIFT) + lane; t < s1; t += WAVE)
+      gstore<DT, false>(a.samp_out_keys, a.samp_out_vals, t,
+                        gload<DT, false>(a.out_keys, a.out_vals, t << SAMP_SHIFT));
   }
 }
 
