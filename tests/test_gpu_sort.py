@@ -187,6 +187,34 @@ def test_merge_round_matches_oracle(gmms, oracle, dev):
     assert np.array_equal(_host(tout), oracle.merge_round_i32(b, K, 1024))
 
 
+@pytest.mark.parametrize("L", [1000, 2048])
+def test_merge_round_pairs_any_list_length(gmms, oracle, dev, L):
+    """A merge round over lists of L elements: L = 2048 lets the partition
+    search the 256-stride sample array (keys and values), L = 1000 does not
+    (lists do not start at multiples of 256). Reference: each task of K
+    lists sorted by (key, value) on the CPU."""
+    import torch
+
+    n, K = 150_007, 8
+    print(f"merge_round pairs L={L}", flush=True)
+    k, v = oracle.gen_pairs(23 + L, n, 0)
+    k = k >> np.uint64(44)  # duplicate keys: ties broken by value
+    for s0 in range(0, n, L):  # sorted input lists
+        o = np.lexsort((v[s0:s0 + L], k[s0:s0 + L]))
+        k[s0:s0 + L] = k[s0:s0 + L][o]
+        v[s0:s0 + L] = v[s0:s0 + L][o]
+    want_k, want_v = k.copy(), v.copy()
+    for s0 in range(0, n, K * L):
+        o = np.lexsort((v[s0:s0 + K * L], k[s0:s0 + K * L]))
+        want_k[s0:s0 + K * L] = k[s0:s0 + K * L][o]
+        want_v[s0:s0 + K * L] = v[s0:s0 + K * L][o]
+    tk, tv = _to_dev(k.view(np.int64), dev), _to_dev(v.view(np.int32), dev)
+    ok_, ov_ = torch.empty_like(tk), torch.empty_like(tv)
+    gmms.merge_round(tk, ok_, L, tv, ov_, dtype=gmms.PAIR_U64_U32, params=gmms.make_params(K=K))
+    assert np.array_equal(_host(ok_).view(np.uint64), want_k)
+    assert np.array_equal(_host(ov_).view(np.uint32), want_v)
+
+
 def test_multimergesort_dropin_returns_result_buffer(gmms, oracle, dev):
     import torch
 
