@@ -153,9 +153,10 @@ template <int DT, bool CONV_IN, bool CONV_OUT, bool RUN4 = false>
 __global__ __launch_bounds__(BASE_WAVES * WAVE) void k_basecase(void* keys, uint32_t* vals, int64_t n,
                                                                uint32_t flip) {
   using E = typename DType<DT>::E;
-  __shared__ uint32_t xp[BASE_WAVES * E::W * XPOSE_WORDS];
+  constexpr bool VEC = E::W == 1;  // 32-bit keys: 16-byte loads and stores
+  __shared__ __align__(16) uint32_t xp[BASE_WAVES * E::W * XPOSE_WORDS];
   const int lane = threadIdx.x & (WAVE - 1);
-  const int wave = threadIdx.x / WAVE;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
   const int64_t tile = (int64_t)blockIdx.x * BASE_WAVES + wave;
   const int64_t base = tile * TILE;
   // without RUN4 there is no workgroup barrier below and an idle wave leaves;
@@ -165,10 +166,23 @@ __global__ __launch_bounds__(BASE_WAVES * WAVE) void k_basecase(void* keys, uint
   const int cnt = rest <= 0 ? 0 : (int)(rest < TILE ? rest : TILE);
 
   E v[TPL];
+  bool loaded = false;
+  if constexpr (VEC) {
+    if (cnt == TILE) {
+      // a full tile of 32-bit keys: registers 4k..4k+3 of lane l are
+      // elements 256k + 4l .. 256k + 4l + 3 (any placement works: the tile
+      // is sorted as a whole)
 #pragma unroll
-  for (int r = 0; r < TPL; ++r) {
-    const int idx = r * WAVE + lane;
-    v[r] = idx < cnt ? gload<DT, CONV_IN>(keys, vals, base + idx, flip) : Limits<E>::hi();
+      for (int k = 0; k < TPL / 4; ++k) gload4<DT, CONV_IN>(keys, base, 256 * k + 4 * lane, &v[4 * k], flip);
+      loaded = true;
+    }
+  }
+  if (!loaded) {
+#pragma unroll
+    for (int r = 0; r < TPL; ++r) {
+      const int idx = r * WAVE + lane;
+      v[r] = idx < cnt ? gload<DT, CONV_IN>(keys, vals, base + idx, flip) : Limits<E>::hi();
+    }
   }
 
   sort_tile(v, lane);
@@ -190,22 +204,53 @@ __global__ __launch_bounds__(BASE_WAVES * WAVE) void k_basecase(void* keys, uint
 
   // transpose through LDS: lane l's registers hold ranks 16l..16l+15
   uint32_t* my = xp + wave * E::W * XPOSE_WORDS;
+  if constexpr (VEC) {
+    // 16-byte chunks, chunk k of lane l at chunk position k ^ xswz(l) of its
+    // 16-word row (no bank conflicts); lane l then reads and stores the four
+    // consecutive ranks 256u + 4l .. + 3 of each row u
 #pragma unroll
-  for (int w = 0; w < E::W; ++w) {
+    for (int k = 0; k < TPL / 4; ++k) {
+      const u32x4 c = {v[4 * k].k, v[4 * k + 1].k, v[4 * k + 2].k, v[4 * k + 3].k};
+      *(u32x4*)(my + lane * TPL + 4 * (k ^ xswz<4>(lane))) = c;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int L = lane / 4;  // the lane whose row holds ranks 4*lane .. 4*lane + 3
+    const int rb = L * TPL + 4 * ((lane & 3) ^ xswz<4>(L));
 #pragma unroll
-    for (int r = 0; r < TPL; ++r) my[w * XPOSE_WORDS + lane * XPOSE_STRIDE + r] = word(v[r], w);
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int u = 0; u < TPL / 4; ++u) {
+      const u32x4 c = *(const u32x4*)(my + u * 256 + rb);
+      E e4[4];
 #pragma unroll
-  for (int r = 0; r < TPL; ++r) {
-    const int e = r * WAVE + lane;  // rank
-    const int slot = (e / TPL) * XPOSE_STRIDE + (e % TPL);
-    uint32_t w[E::W];
+      for (int t = 0; t < 4; ++t) e4[t].k = c[t];
+      const int o = u * 256 + 4 * lane;
+      if (cnt == TILE) {
+        gstore4<DT, CONV_OUT>(keys, vals, base, o, e4, flip);
+      } else {
 #pragma unroll
-    for (int q = 0; q < E::W; ++q) w[q] = my[q * XPOSE_WORDS + slot];
-    if (e < cnt) gstore<DT, CONV_OUT>(keys, vals, base + e, from_words<E>(w), flip);
+        for (int t = 0; t < 4; ++t)
+          if (o + t < cnt) gstore<DT, CONV_OUT>(keys, vals, base + o + t, e4[t], flip);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int w = 0; w < E::W; ++w) {
+#pragma unroll
+      for (int r = 0; r < TPL; ++r) my[w * XPOSE_WORDS + lane * XPOSE_STRIDE + r] = word(v[r], w);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int r = 0; r < TPL; ++r) {
+      const int e = r * WAVE + lane;  // rank
+      const int slot = (e / TPL) * XPOSE_STRIDE + (e % TPL);
+      uint32_t w[E::W];
+#pragma unroll
+      for (int q = 0; q < E::W; ++q) w[q] = my[q * XPOSE_WORDS + slot];
+      if (e < cnt) gstore<DT, CONV_OUT>(keys, vals, base + e, from_words<E>(w), flip);
+    }
   }
 }
 

@@ -49,6 +49,27 @@ __device__ __forceinline__ uint32_t xor_lane(uint32_t x) {
   else return (uint32_t)__shfl_xor((int)x, M, 64);
 }
 
+// a wave-uniform 64-bit value, held in scalar registers
+__device__ __forceinline__ int64_t uniform64(int64_t x) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)x >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Chunk swizzle of the register-tile transpose (XC = 4 chunks of 4 words
+// per lane row of 16 words): a lane's chunk k goes to chunk position
+// k ^ xswz(lane). Lanes 0,2,4,6 (and 1,3,5,7) get 0,1,3,2, and lanes m, m+4,
+// m+8, m+12 get four distinct values, so the 16-byte accesses of 8
+// consecutive lanes (32 banks) or 16 consecutive lanes (64 banks) touch every
+// bank once.
+template <int XC>
+__device__ __forceinline__ int xswz(int l) {
+  static_assert(XC == 4, "rows of 16 words");
+  return ((l >> 1) ^ (l >> 2)) & 3;
+}
+
 // ---------------------------------------------------------------- elements
 struct E32 {
   static constexpr int W = 1;
@@ -231,6 +252,18 @@ __device__ __forceinline__ void gstore(void* keys, uint32_t* vals, int64_t i, co
   if (CONV) x = D::out(x ^ (Wd)(int32_t)flip);
   static_cast<Wd*>(keys)[i] = x;
   if constexpr (E::HAS_VAL) vals[i] = CONV ? e.v ^ flip : e.v;
+}
+// Four consecutive 32-bit elements base + o .. base + o + 3 into e[0..3]
+// with one 16-byte load (element alignment suffices).
+template <int DT, bool CONV>
+__device__ __forceinline__ void gload4(const void* keys, int64_t base, int o, typename DType<DT>::E* e,
+                                       uint32_t flip = 0u) {
+  using D = DType<DT>;
+  static_assert(D::E::W == 1, "32-bit keys");
+  typedef uint32_t U4 __attribute__((ext_vector_type(4), aligned(4)));
+  const U4 x = *(const U4*)(static_cast<const uint32_t*>(keys) + base + o);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) e[t].k = CONV ? D::in(x[t]) ^ flip : x[t];
 }
 // Four consecutive elements e[0..3] to positions base + o .. base + o + 3
 // with 16-byte vector stores (two for 64-bit keys). base is wave-uniform and

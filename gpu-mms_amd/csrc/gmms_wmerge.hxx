@@ -43,27 +43,6 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// a wave-uniform 64-bit value, held in scalar registers
-__device__ __forceinline__ int64_t uniform64(int64_t x) {
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)x >> 32));
-  return (int64_t)(((uint64_t)hi << 32) | lo);
-}
-
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-
-// Chunk swizzle of the register-tile transpose (XC = 4 chunks of 4 words
-// per lane row of 16 words): a lane's chunk k goes to chunk position
-// k ^ xswz(lane). Lanes 0,2,4,6 (and 1,3,5,7) get 0,1,3,2, and lanes m, m+4,
-// m+8, m+12 get four distinct values, so the 16-byte accesses of 8
-// consecutive lanes (32 banks) or 16 consecutive lanes (64 banks) touch every
-// bank once.
-template <int XC>
-__device__ __forceinline__ int xswz(int l) {
-  static_assert(XC == 4, "rows of 16 words");
-  return ((l >> 1) ^ (l >> 2)) & 3;
-}
-
 template <class E>
 __device__ __forceinline__ E wshfl(const E& e, int src) {
   uint32_t w[E::W];
