@@ -144,67 +144,36 @@ __device__ __forceinline__ void wave_exchange(E (&v)[TPL], uint32_t* buf, int wa
   }
 }
 
-// In-place base case over keys[0..n): CONV_IN maps raw -> internal on load,
-// CONV_OUT maps internal -> raw on store. RUN4: the workgroup's four sorted
-// tiles are merged into one sorted run of 4096 (two more merge levels; the
-// flip and distance-1024 stages go through LDS between wavefronts, the rest
-// are the in-wavefront stages of merge_level).
-template <int DT, bool CONV_IN, bool CONV_OUT, bool RUN4 = false>
-__global__ __launch_bounds__(BASE_WAVES * WAVE) void k_basecase(void* keys, uint32_t* vals, int64_t n,
-                                                               uint32_t flip) {
+// Tile load: elements base .. base + cnt - 1 into v (the maximum element
+// past cnt). A full tile of 32-bit keys loads 16 bytes per lane: registers
+// 4k..4k+3 of lane l are elements 256k + 4l .. + 3 (any placement works:
+// the tile is sorted as a whole).
+template <int DT, bool CONV>
+__device__ __forceinline__ void tile_load(const void* keys, const uint32_t* vals, int64_t base, int cnt,
+                                          uint32_t flip, typename DType<DT>::E (&v)[TPL], int lane) {
   using E = typename DType<DT>::E;
-  constexpr bool VEC = E::W == 1;  // 32-bit keys: 16-byte loads and stores
-  __shared__ __align__(16) uint32_t xp[BASE_WAVES * E::W * XPOSE_WORDS];
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
-  const int64_t tile = (int64_t)blockIdx.x * BASE_WAVES + wave;
-  const int64_t base = tile * TILE;
-  // without RUN4 there is no workgroup barrier below and an idle wave leaves;
-  // with it, a wave past the end sorts a tile of maxima and stores nothing
-  if (!RUN4 && base >= n) return;
-  const int64_t rest = n - base;
-  const int cnt = rest <= 0 ? 0 : (int)(rest < TILE ? rest : TILE);
-
-  E v[TPL];
-  bool loaded = false;
-  if constexpr (VEC) {
+  if constexpr (E::W == 1) {
     if (cnt == TILE) {
-      // a full tile of 32-bit keys: registers 4k..4k+3 of lane l are
-      // elements 256k + 4l .. 256k + 4l + 3 (any placement works: the tile
-      // is sorted as a whole)
 #pragma unroll
-      for (int k = 0; k < TPL / 4; ++k) gload4<DT, CONV_IN>(keys, base, 256 * k + 4 * lane, &v[4 * k], flip);
-      loaded = true;
+      for (int k = 0; k < TPL / 4; ++k) gload4<DT, CONV>(keys, base, 256 * k + 4 * lane, &v[4 * k], flip);
+      return;
     }
   }
-  if (!loaded) {
 #pragma unroll
-    for (int r = 0; r < TPL; ++r) {
-      const int idx = r * WAVE + lane;
-      v[r] = idx < cnt ? gload<DT, CONV_IN>(keys, vals, base + idx, flip) : Limits<E>::hi();
-    }
+  for (int r = 0; r < TPL; ++r) {
+    const int idx = r * WAVE + lane;
+    v[r] = idx < cnt ? gload<DT, CONV>(keys, vals, base + idx, flip) : Limits<E>::hi();
   }
+}
 
-  sort_tile(v, lane);
-
-  if constexpr (RUN4) {
-    static_assert(BASE_WAVES == 4, "a run of 4096 is four tiles");
-    // 1024 -> 2048: flip across wavefront pairs, then the in-wavefront cleaners
-    wave_exchange<true>(v, xp, wave, wave ^ 1, (wave & 1) == 0, lane);
-    lane_cleaners<6>(v, lane);
-    clean_regs(v);
-    // 2048 -> 4096: flip across the two halves, the cleaner at distance 1024,
-    // then the in-wavefront cleaners
-    wave_exchange<true>(v, xp, wave, 3 - wave, wave < 2, lane);
-    wave_exchange<false>(v, xp, wave, wave ^ 1, (wave & 1) == 0, lane);
-    lane_cleaners<6>(v, lane);
-    clean_regs(v);
-    __syncthreads();  // xp is reused by the store transpose below
-  }
-
-  // transpose through LDS: lane l's registers hold ranks 16l..16l+15
-  uint32_t* my = xp + wave * E::W * XPOSE_WORDS;
-  if constexpr (VEC) {
+// Tile store of the sorted tile (lane l holds ranks 16l .. 16l+15) to
+// base .. base + cnt - 1 through the wavefront's LDS region `my`
+// (E::W * XPOSE_WORDS words), so that the stores are coalesced.
+template <int DT, bool CONV>
+__device__ __forceinline__ void tile_store(void* keys, uint32_t* vals, int64_t base, int cnt, uint32_t flip,
+                                           const typename DType<DT>::E (&v)[TPL], uint32_t* my, int lane) {
+  using E = typename DType<DT>::E;
+  if constexpr (E::W == 1) {
     // 16-byte chunks, chunk k of lane l at chunk position k ^ xswz(l) of its
     // 16-word row (no bank conflicts); lane l then reads and stores the four
     // consecutive ranks 256u + 4l .. + 3 of each row u
@@ -226,11 +195,11 @@ __global__ __launch_bounds__(BASE_WAVES * WAVE) void k_basecase(void* keys, uint
       for (int t = 0; t < 4; ++t) e4[t].k = c[t];
       const int o = u * 256 + 4 * lane;
       if (cnt == TILE) {
-        gstore4<DT, CONV_OUT>(keys, vals, base, o, e4, flip);
+        gstore4<DT, CONV>(keys, vals, base, o, e4, flip);
       } else {
 #pragma unroll
         for (int t = 0; t < 4; ++t)
-          if (o + t < cnt) gstore<DT, CONV_OUT>(keys, vals, base + o + t, e4[t], flip);
+          if (o + t < cnt) gstore<DT, CONV>(keys, vals, base + o + t, e4[t], flip);
       }
     }
   } else {
@@ -249,9 +218,52 @@ __global__ __launch_bounds__(BASE_WAVES * WAVE) void k_basecase(void* keys, uint
       uint32_t w[E::W];
 #pragma unroll
       for (int q = 0; q < E::W; ++q) w[q] = my[q * XPOSE_WORDS + slot];
-      if (e < cnt) gstore<DT, CONV_OUT>(keys, vals, base + e, from_words<E>(w), flip);
+      if (e < cnt) gstore<DT, CONV>(keys, vals, base + e, from_words<E>(w), flip);
     }
   }
+}
+
+// In-place base case over keys[0..n): CONV_IN maps raw -> internal on load,
+// CONV_OUT maps internal -> raw on store. RUN4: the workgroup's four sorted
+// tiles are merged into one sorted run of 4096 (two more merge levels; the
+// flip and distance-1024 stages go through LDS between wavefronts, the rest
+// are the in-wavefront stages of merge_level).
+template <int DT, bool CONV_IN, bool CONV_OUT, bool RUN4 = false>
+__global__ __launch_bounds__(BASE_WAVES * WAVE) void k_basecase(void* keys, uint32_t* vals, int64_t n,
+                                                               uint32_t flip) {
+  using E = typename DType<DT>::E;
+  __shared__ __align__(16) uint32_t xp[BASE_WAVES * E::W * XPOSE_WORDS];
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+  const int64_t tile = (int64_t)blockIdx.x * BASE_WAVES + wave;
+  const int64_t base = tile * TILE;
+  // without RUN4 there is no workgroup barrier below and an idle wave leaves;
+  // with it, a wave past the end sorts a tile of maxima and stores nothing
+  if (!RUN4 && base >= n) return;
+  const int64_t rest = n - base;
+  const int cnt = rest <= 0 ? 0 : (int)(rest < TILE ? rest : TILE);
+
+  E v[TPL];
+  tile_load<DT, CONV_IN>(keys, vals, base, cnt, flip, v, lane);
+
+  sort_tile(v, lane);
+
+  if constexpr (RUN4) {
+    static_assert(BASE_WAVES == 4, "a run of 4096 is four tiles");
+    // 1024 -> 2048: flip across wavefront pairs, then the in-wavefront cleaners
+    wave_exchange<true>(v, xp, wave, wave ^ 1, (wave & 1) == 0, lane);
+    lane_cleaners<6>(v, lane);
+    clean_regs(v);
+    // 2048 -> 4096: flip across the two halves, the cleaner at distance 1024,
+    // then the in-wavefront cleaners
+    wave_exchange<true>(v, xp, wave, 3 - wave, wave < 2, lane);
+    wave_exchange<false>(v, xp, wave, wave ^ 1, (wave & 1) == 0, lane);
+    lane_cleaners<6>(v, lane);
+    clean_regs(v);
+    __syncthreads();  // xp is reused by the store transpose below
+  }
+
+  tile_store<DT, CONV_OUT>(keys, vals, base, cnt, flip, v, xp + wave * E::W * XPOSE_WORDS, lane);
 }
 
 // Batched segments: run descriptors instead of one array. A descriptor holds
@@ -273,9 +285,9 @@ __global__ __launch_bounds__(BASE_WAVES* WAVE) void k_basecase_seg(void* k0, uin
                                                                     const TileDesc* tiles, int64_t n_tiles,
                                                                     uint32_t flip) {
   using E = typename DType<DT>::E;
-  __shared__ uint32_t xp[BASE_WAVES * E::W * XPOSE_WORDS];
+  __shared__ __align__(16) uint32_t xp[BASE_WAVES * E::W * XPOSE_WORDS];
   const int lane = threadIdx.x & (WAVE - 1);
-  const int wave = threadIdx.x / WAVE;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
   const int64_t tile = RUN4 ? (int64_t)blockIdx.x : (int64_t)blockIdx.x * BASE_WAVES + wave;
   if (tile >= n_tiles) return;  // RUN4: the whole workgroup leaves together
   const TileDesc td = tiles[tile];
@@ -284,11 +296,7 @@ __global__ __launch_bounds__(BASE_WAVES* WAVE) void k_basecase_seg(void* k0, uin
   const int cnt = rest <= 0 ? 0 : (rest < TILE ? rest : TILE);
 
   E v[TPL];
-#pragma unroll
-  for (int r = 0; r < TPL; ++r) {
-    const int idx = r * WAVE + lane;
-    v[r] = idx < cnt ? gload<DT, true>(k0, v0, base + idx, flip) : Limits<E>::hi();
-  }
+  tile_load<DT, true>(k0, v0, base, cnt, flip, v, lane);
 
   sort_tile(v, lane);
 
@@ -304,29 +312,10 @@ __global__ __launch_bounds__(BASE_WAVES* WAVE) void k_basecase_seg(void* k0, uin
   }
 
   uint32_t* my = xp + wave * E::W * XPOSE_WORDS;
-#pragma unroll
-  for (int w = 0; w < E::W; ++w) {
-#pragma unroll
-    for (int r = 0; r < TPL; ++r) my[w * XPOSE_WORDS + lane * XPOSE_STRIDE + r] = word(v[r], w);
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   void* dk = (td.flags & 1) ? k1 : k0;
   uint32_t* dv = (td.flags & 1) ? v1 : v0;
-  const bool conv = (td.flags & 2) != 0;
-#pragma unroll
-  for (int r = 0; r < TPL; ++r) {
-    const int e = r * WAVE + lane;
-    const int slot = (e / TPL) * XPOSE_STRIDE + (e % TPL);
-    uint32_t w[E::W];
-#pragma unroll
-    for (int q = 0; q < E::W; ++q) w[q] = my[q * XPOSE_WORDS + slot];
-    if (e < cnt) {
-      if (conv) gstore<DT, true>(dk, dv, base + e, from_words<E>(w), flip);
-      else gstore<DT, false>(dk, dv, base + e, from_words<E>(w));
-    }
-  }
+  if (td.flags & 2) tile_store<DT, true>(dk, dv, base, cnt, flip, v, my, lane);
+  else tile_store<DT, false>(dk, dv, base, cnt, 0u, v, my, lane);
 }
 
 }  // namespace gmms
