@@ -125,6 +125,33 @@ __device__ __forceinline__ void wave_exchange(E (&v)[TPL], uint32_t* buf, int wa
   constexpr int W = E::W;
   __syncthreads();  // the previous use of buf is over
   uint32_t* mine = buf + wave * W * XPOSE_WORDS;
+  if constexpr (W == 1) {
+    // 32-bit keys: rows of 16 words in swizzled 16-byte chunks (chunk k of
+    // row l at chunk position k ^ xswz(l)), read back 16 bytes at a time
+#pragma unroll
+    for (int k = 0; k < TPL / 4; ++k) {
+      const u32x4 c = {v[4 * k].k, v[4 * k + 1].k, v[4 * k + 2].k, v[4 * k + 3].k};
+      *(u32x4*)(mine + lane * TPL + 4 * (k ^ xswz<4>(lane))) = c;
+    }
+    __syncthreads();
+    const uint32_t* theirs = buf + partner * W * XPOSE_WORDS;
+    const int pl = MIRROR ? WAVE - 1 - lane : lane;
+    const uint32_t hsel = low ? 0u : 0xFFFFFFFFu;
+#pragma unroll
+    for (int k = 0; k < TPL / 4; ++k) {
+      // MIRROR: register 4k+t meets the partner's register 15-4k-t, the
+      // reversed chunk 3-k
+      const int pk = MIRROR ? TPL / 4 - 1 - k : k;
+      const u32x4 c = *(const u32x4*)(theirs + pl * TPL + 4 * (pk ^ xswz<4>(pl)));
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        E p;
+        p.k = c[MIRROR ? 3 - t : t];
+        v[4 * k + t] = lane_ce(v[4 * k + t], p, low, hsel);
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int w = 0; w < W; ++w) {
 #pragma unroll
