@@ -229,6 +229,30 @@ def test_multimergesort_dropin_returns_result_buffer(gmms, oracle, dev):
         assert np.array_equal(_host(res), oracle.sort(a))
 
 
+def test_buffers_off_16_byte_alignment(gmms, oracle, dev):
+    """Key buffers that start 4 bytes past a 16-byte boundary (views at
+    element offset 1): the 16-byte loads and stores of the base case and the
+    merge rounds need only element alignment. The guard elements around the
+    views stay untouched."""
+    import torch
+
+    n = 3 * 65536 + 4097
+    print("unaligned buffers", flush=True)
+    a = oracle.gen_i32(41, n, 34)
+    big = torch.full((n + 2,), -7, dtype=torch.int32, device=dev)
+    alt = torch.full((n + 2,), -9, dtype=torch.int32, device=dev)
+    big[1:n + 1] = _to_dev(a, dev)
+    k, ka = big[1:n + 1], alt[1:n + 1]
+    assert k.data_ptr() % 16 == 4
+    which = gmms.sort_inplace(k, ka)
+    torch.cuda.synchronize()
+    got = _host(ka if which else k)
+    assert np.array_equal(got, oracle.sort(a))
+    for t, guard in ((big, -7), (alt, -9)):
+        h = _host(t)
+        assert h[0] == guard and h[-1] == guard
+
+
 def test_pad_writes_maximum(gmms, dev):
     import torch
 
