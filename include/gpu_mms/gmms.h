@@ -99,10 +99,11 @@ int gmms_num_rounds(int64_t n, int K, int* rounds);
  * gmms_num_rounds whenever the 4096-element base runs save one). */
 int gmms_sort_rounds(int dtype, int64_t n, const gmms_params* p, int* rounds);
 
-/* Device workspace (partition cuts) that gmms_sort needs for n elements.
- * The reference cudaMalloc's its pivot array inside every call
- * (multimergesort.hxx:61-63,138); here the caller owns it so the timed
- * path never allocates. */
+/* Device workspace that gmms_sort needs for n elements: the partition's
+ * sample array (every 256th element of a round's input) and its cut rows;
+ * 0 when no round splits a task. The reference cudaMalloc's its pivot array
+ * inside every call (multimergesort.hxx:61-63,138); here the caller owns it
+ * so the timed path never allocates. */
 int gmms_workspace_size(int dtype, int64_t n, const gmms_params* p, size_t* bytes);
 /* Workspace for one gmms_merge_round over n elements in lists of list_len. */
 int gmms_round_workspace_size(int dtype, int64_t n, int64_t list_len, const gmms_params* p,
