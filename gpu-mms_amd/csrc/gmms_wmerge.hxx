@@ -284,52 +284,43 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
         int kz;
         const bool ne = live && want > 0 && lane != ky;
         frontier_min<E>(ne ? lget<E>(win, WIN, lane * F + head) : Limits<E>::hi(), ne ? lane : WAVE, Kp, z, kz);
-        // run of list ky (from cur + F) that precedes z
-        const int64_t base = __shfl(cur, ky, WAVE);
-        int64_t glo = F, ghi = __shfl(endp, ky, WAVE) - base;
-        if (kz != WAVE) {
-          while (ghi > glo) {
-            const int64_t span = ghi - glo;
-            const int64_t step = (span + WAVE - 1) / WAVE;
-            const int64_t p = glo + (int64_t)lane * step;
-            bool before = true;
-            if (p < ghi) {
-              const E v = gload<DT, false>(a.in_keys, a.in_vals, base + p);
-              before = !lt(z, v);  // v <= z
-            }
-            const unsigned long long nb = __ballot(!before && p < ghi);
-            if (nb == 0) {
-              const int64_t np = (ghi - 1 - glo) / step;
-              glo = glo + (np < WAVE - 1 ? np : WAVE - 1) * step + 1;
-            } else {
-              const int l0 = __ffsll((long long)nb) - 1;
-              const int64_t phi = glo + (int64_t)l0 * step;
-              glo = l0 == 0 ? glo : glo + (int64_t)(l0 - 1) * step + 1;
-              ghi = phi;
-            }
+        // list ky's run, from cur, that precedes z goes straight from input
+        // to output: 4 elements per lane in flight, each batch checked
+        // against z, up to the first element past z (the run is sorted, so
+        // the check holds for a prefix) or the end of the list. One memory
+        // round trip per 256 elements and none to find the end first.
+        const int64_t src = uniform64(__shfl(cur, ky, WAVE));
+        const int64_t lim = uniform64(__shfl(endp, ky, WAVE)) - src;
+        int64_t pos = 0;
+        for (;;) {
+          E g[4];
+          bool ok[4];
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const int64_t idx = pos + t * WAVE + lane;
+            const bool valid = idx < lim;
+            g[t] = valid ? gload<DT, false>(a.in_keys, a.in_vals, src + idx) : Limits<E>::hi();
+            ok[t] = valid && (kz == WAVE || !lt(z, g[t]));  // g <= z
           }
-        } else {
-          glo = ghi;
+          int f = 4 * WAVE;  // batch index (t*64 + lane order) of the first element not taken
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const unsigned long long nb = __ballot(!ok[t]);
+            if (f == 4 * WAVE && nb != 0) f = t * WAVE + __ffsll((long long)nb) - 1;
+          }
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+            if (t * WAVE + lane < f)
+              gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + pos + t * WAVE + lane, g[t], a.flip);
+          pos += f;
+          if (f < 4 * WAVE) break;
         }
-        glen = uniform64(glo);
+        glen = pos;
         sk = 0;
       }
     }
 
     if (gallop) {
-      // list ky's run goes straight from input to output, 4 elements per lane in flight
-      const int64_t src = __shfl(cur, ky, WAVE);
-      int64_t i = lane;
-      for (; i + 3 * WAVE < glen; i += 4 * WAVE) {
-        E g[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) g[t] = gload<DT, false>(a.in_keys, a.in_vals, src + i + t * WAVE);
-#pragma unroll
-        for (int t = 0; t < 4; ++t) gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + i + t * WAVE, g[t], a.flip);
-      }
-      for (; i < glen; i += WAVE)
-        gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + i, gload<DT, false>(a.in_keys, a.in_vals, src + i),
-                             a.flip);
       // list ky restarts with an empty window; the others keep theirs
       int c2 = want;
       if (lane == ky) {
