@@ -175,7 +175,7 @@ RoundArgs round_args(const void* ik, const void* iv, void* ok, void* ov, int64_t
 // (every SAMP-th element, at the start of ws) when the lists start at
 // multiples of SAMP: gathered here unless `samp_in` says the previous round
 // wrote it. `samp_out`: the wavefront merger writes the samples of its
-// output for the next round (the return value says whether it did).
+// output for the next round (*wrote says whether it did).
 template <int DT>
 int run_round(const void* ik, const void* iv, void* ok, void* ov, int64_t n, int64_t L, const Resolved& r,
               bool cout, void* ws, size_t wsb, hipStream_t s, bool samp_in = false, bool samp_out = false,

@@ -221,7 +221,9 @@ __device__ __forceinline__ void partition_one(const RoundArgs& a, const MergerIn
         }
       }
     }
-    const E x = !act ? Limits<E>::hi() : from_samp ? ld_samp<DT>(a, (lbase + pos) >> SAMP_SHIFT) : ld_int<DT>(a, lbase + pos);
+    const E x = !act       ? Limits<E>::hi()
+                : from_samp ? ld_samp<DT>(a, (lbase + pos) >> SAMP_SHIFT)
+                            : ld_int<DT>(a, lbase + pos);
     // rank of this candidate among active candidates, order (element,
     // lane); the candidates are read with readlane (no LDS round trips)
     const unsigned long long amask = __ballot(act);
