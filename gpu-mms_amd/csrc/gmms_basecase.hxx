@@ -172,9 +172,9 @@ __device__ __forceinline__ void wave_exchange(E (&v)[TPL], uint32_t* buf, int wa
 }
 
 // Tile load: elements base .. base + cnt - 1 into v (the maximum element
-// past cnt). A full tile of 32-bit keys loads 16 bytes per lane: registers
-// 4k..4k+3 of lane l are elements 256k + 4l .. + 3 (any placement works:
-// the tile is sorted as a whole).
+// past cnt). A full tile loads 16 bytes of keys per lane and load: for
+// 32-bit keys registers 4k..4k+3 of lane l are elements 256k + 4l .. + 3
+// (any placement works: the tile is sorted as a whole).
 template <int DT, bool CONV>
 __device__ __forceinline__ void tile_load(const void* keys, const uint32_t* vals, int64_t base, int cnt,
                                           uint32_t flip, typename DType<DT>::E (&v)[TPL], int lane) {
@@ -183,6 +183,14 @@ __device__ __forceinline__ void tile_load(const void* keys, const uint32_t* vals
     if (cnt == TILE) {
 #pragma unroll
       for (int k = 0; k < TPL / 4; ++k) gload4<DT, CONV>(keys, base, 256 * k + 4 * lane, &v[4 * k], flip);
+      return;
+    }
+  } else {
+    // 64-bit keys: two elements per 16-byte load, registers 2k, 2k+1 of
+    // lane l are elements 128k + 2l, 128k + 2l + 1
+    if (cnt == TILE) {
+#pragma unroll
+      for (int k = 0; k < TPL / 2; ++k) gload2<DT, CONV>(keys, vals, base, 128 * k + 2 * lane, &v[2 * k], flip);
       return;
     }
   }

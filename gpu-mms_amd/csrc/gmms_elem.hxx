@@ -265,6 +265,27 @@ __device__ __forceinline__ void gload4(const void* keys, int64_t base, int o, ty
 #pragma unroll
   for (int t = 0; t < 4; ++t) e[t].k = CONV ? D::in(x[t]) ^ flip : x[t];
 }
+// Two consecutive elements of 64-bit keys (and their values) base + o,
+// base + o + 1 into e[0..1] with one 16-byte key load (and one 8-byte value
+// load); element alignment suffices.
+template <int DT, bool CONV>
+__device__ __forceinline__ void gload2(const void* keys, const uint32_t* vals, int64_t base, int o,
+                                       typename DType<DT>::E* e, uint32_t flip = 0u) {
+  using D = DType<DT>;
+  using E = typename D::E;
+  using Wd = typename D::Word;
+  static_assert(sizeof(Wd) == 8, "64-bit keys");
+  typedef Wd K2 __attribute__((ext_vector_type(2), aligned(8)));
+  const K2 x = *(const K2*)(static_cast<const Wd*>(keys) + base + o);
+#pragma unroll
+  for (int t = 0; t < 2; ++t) e[t].k = CONV ? D::in(x[t]) ^ (Wd)(int32_t)flip : x[t];
+  if constexpr (E::HAS_VAL) {
+    typedef uint32_t V2 __attribute__((ext_vector_type(2), aligned(4)));
+    const V2 y = *(const V2*)(vals + base + o);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) e[t].v = CONV ? y[t] ^ flip : y[t];
+  }
+}
 // Four consecutive elements e[0..3] to positions base + o .. base + o + 3
 // with 16-byte vector stores (two for 64-bit keys). base is wave-uniform and
 // the addresses need only element alignment.
