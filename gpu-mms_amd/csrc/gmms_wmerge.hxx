@@ -20,6 +20,7 @@
 // largest possible real element bit for bit, so a real maximum and a filler
 // are interchangeable and the first S slots are exact.
 #pragma once
+#include <type_traits>
 #include "gmms_basecase.hxx"
 #include "gmms_elem.hxx"
 #include "gmms_merge.hxx"
@@ -44,20 +45,6 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 template <class E>
-__device__ __forceinline__ E wshfl(const E& e, int src) {
-  uint32_t w[E::W];
-#pragma unroll
-  for (int q = 0; q < E::W; ++q) w[q] = (uint32_t)__shfl((int)word(e, q), src, WAVE);
-  return from_words<E>(w);
-}
-template <class E>
-__device__ __forceinline__ E wxor(const E& e, int m) {
-  uint32_t w[E::W];
-#pragma unroll
-  for (int q = 0; q < E::W; ++q) w[q] = (uint32_t)__shfl_xor((int)word(e, q), m, WAVE);
-  return from_words<E>(w);
-}
-template <class E>
 __device__ __forceinline__ E lget(const uint32_t* base, int cap, int idx) {
   uint32_t w[E::W];
 #pragma unroll
@@ -74,16 +61,23 @@ __device__ __forceinline__ void lput(uint32_t* base, int cap, int idx, const E& 
 // lanes without a candidate pass (max element, WAVE). k = WAVE: none.
 template <class E>
 __device__ __forceinline__ void frontier_min(E x, int k, int Kp, E& y, int& ky) {
-#pragma unroll
-  for (int d = 1; d < WAVE; d <<= 1) {
-    if (d < Kp) {
-      const E ox = wxor(x, d);
-      const int ok = __shfl_xor(k, d, WAVE);
+  // butterfly over lane distances 1..Kp/2 with the DPP / swizzle exchanges
+  auto lvl = [&](auto dc) {
+    constexpr int D = decltype(dc)::value;
+    if (D < Kp) {
+      const E ox = xlane<D>(x);
+      const int ok = (int)xor_lane<D>((uint32_t)k);
       const bool take = ok < WAVE && (k == WAVE || lt(ox, x) || (!lt(x, ox) && ok < k));
       x = sel(take, ox, x);
       k = take ? ok : k;
     }
-  }
+  };
+  lvl(std::integral_constant<int, 1>{});
+  lvl(std::integral_constant<int, 2>{});
+  lvl(std::integral_constant<int, 4>{});
+  lvl(std::integral_constant<int, 8>{});
+  lvl(std::integral_constant<int, 16>{});
+  lvl(std::integral_constant<int, 32>{});
   uint32_t w[E::W];
 #pragma unroll
   for (int q = 0; q < E::W; ++q) w[q] = (uint32_t)__builtin_amdgcn_readfirstlane((int)word(x, q));
@@ -94,14 +88,20 @@ __device__ __forceinline__ void frontier_min(E x, int k, int Kp, E& y, int& ky) 
 template <>
 __device__ __forceinline__ void frontier_min<E32>(E32 x, int k, int Kp, E32& y, int& ky) {
   uint64_t key = k == WAVE ? ~0ull : (((uint64_t)x.k << 7) | (uint64_t)k);
-#pragma unroll
-  for (int d = 1; d < WAVE; d <<= 1) {
-    if (d < Kp) {
-      const uint64_t o = ((uint64_t)(uint32_t)__shfl_xor((int)(key >> 32), d, WAVE) << 32) |
-                         (uint32_t)__shfl_xor((int)key, d, WAVE);
+  // butterfly over lane distances 1..Kp/2 with the DPP / swizzle exchanges
+  auto lvl = [&](auto dc) {
+    constexpr int D = decltype(dc)::value;
+    if (D < Kp) {
+      const uint64_t o = ((uint64_t)xor_lane<D>((uint32_t)(key >> 32)) << 32) | xor_lane<D>((uint32_t)key);
       key = o < key ? o : key;
     }
-  }
+  };
+  lvl(std::integral_constant<int, 1>{});
+  lvl(std::integral_constant<int, 2>{});
+  lvl(std::integral_constant<int, 4>{});
+  lvl(std::integral_constant<int, 8>{});
+  lvl(std::integral_constant<int, 16>{});
+  lvl(std::integral_constant<int, 32>{});
   const uint64_t kk = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(key >> 32)) << 32) |
                       (uint32_t)__builtin_amdgcn_readfirstlane((int)key);
   ky = kk == ~0ull ? WAVE : (int)(kk & 127u);
