@@ -337,7 +337,17 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
     }
 
     // ---- elements taken this step, and the state after it
-    const int S = __builtin_amdgcn_readfirstlane(wave_sum32(sk));
+    // S: row sums by DPP butterflies, then the four rows' totals (no LDS)
+    int S;
+    {
+      uint32_t t = (uint32_t)sk;
+      t += dpp_mov<0xB1>(t);   // quad_perm [1,0,3,2]
+      t += dpp_mov<0x4E>(t);   // quad_perm [2,3,0,1]
+      t += dpp_mov<0x141>(t);  // row_half_mirror
+      t += dpp_mov<0x140>(t);  // row_mirror
+      S = __builtin_amdgcn_readlane((int)t, 0) + __builtin_amdgcn_readlane((int)t, 16) +
+          __builtin_amdgcn_readlane((int)t, 32) + __builtin_amdgcn_readlane((int)t, 48);
+    }
     const int64_t cur2 = cur + sk;
     const int head2 = (head + sk) & (F - 1);
     const int cnt2 = want - sk;
