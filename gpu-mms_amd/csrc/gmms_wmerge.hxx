@@ -59,14 +59,29 @@ __device__ __forceinline__ void lput(uint32_t* base, int cap, int idx, const E& 
 
 // Wave-uniform minimum of (x, k) over lanes 0..Kp-1 in (element, lane) order;
 // lanes without a candidate pass (max element, WAVE). k = WAVE: none.
+// Partner for level D of a minimum reduction whose result is read from
+// lane 0: after level D, lane 0 must hold the minimum of lanes 0 .. 2D-1.
+// Level 4 reads lane l + 4 (mod 16) with the DPP rotation row_ror:12 (no DPP
+// xor-4 pattern exists), so lane 0 combines lanes 0-3 with 4-7 and lane 8
+// lanes 8-11 with 12-15 for level 8; the other levels use the xor exchange.
+template <int D>
+__device__ __forceinline__ uint32_t rot_lane(uint32_t x) {
+  if constexpr (D == 4) return dpp_mov<0x12C>(x);  // row_ror:12: lane l reads lane (l + 4) & 15
+  else return xor_lane<D>(x);
+}
+
 template <class E>
 __device__ __forceinline__ void frontier_min(E x, int k, int Kp, E& y, int& ky) {
-  // butterfly over lane distances 1..Kp/2 with the DPP / swizzle exchanges
+  // butterfly over lane groups of 2..Kp with the DPP / swizzle exchanges;
+  // lane 0 ends with the minimum
   auto lvl = [&](auto dc) {
     constexpr int D = decltype(dc)::value;
     if (D < Kp) {
-      const E ox = xlane<D>(x);
-      const int ok = (int)xor_lane<D>((uint32_t)k);
+      uint32_t w[E::W];
+#pragma unroll
+      for (int r = 0; r < E::W; ++r) w[r] = rot_lane<D>(word(x, r));
+      const E ox = from_words<E>(w);
+      const int ok = (int)rot_lane<D>((uint32_t)k);
       const bool take = ok < WAVE && (k == WAVE || lt(ox, x) || (!lt(x, ox) && ok < k));
       x = sel(take, ox, x);
       k = take ? ok : k;
@@ -88,11 +103,12 @@ __device__ __forceinline__ void frontier_min(E x, int k, int Kp, E& y, int& ky) 
 template <>
 __device__ __forceinline__ void frontier_min<E32>(E32 x, int k, int Kp, E32& y, int& ky) {
   uint64_t key = k == WAVE ? ~0ull : (((uint64_t)x.k << 7) | (uint64_t)k);
-  // butterfly over lane distances 1..Kp/2 with the DPP / swizzle exchanges
+  // butterfly over lane groups of 2..Kp with the DPP / swizzle exchanges;
+  // lane 0 ends with the minimum
   auto lvl = [&](auto dc) {
     constexpr int D = decltype(dc)::value;
     if (D < Kp) {
-      const uint64_t o = ((uint64_t)xor_lane<D>((uint32_t)(key >> 32)) << 32) | xor_lane<D>((uint32_t)key);
+      const uint64_t o = ((uint64_t)rot_lane<D>((uint32_t)(key >> 32)) << 32) | rot_lane<D>((uint32_t)key);
       key = o < key ? o : key;
     }
   };
