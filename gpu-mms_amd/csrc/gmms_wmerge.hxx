@@ -181,6 +181,8 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
   }
   int64_t obase = uniform64(tbase + wave_sum64(lo));
   const int64_t o0 = obase;  // this merger's output is [o0, final obase)
+  const int64_t total = uniform64(wave_sum64(hi - lo));  // its output count
+  int prio = -1;                                          // issue priority set last
   // compact the non-empty lists, keeping list order
   const bool act = lane < K && hi > lo;
   const unsigned long long amask = __ballot(act);
@@ -257,6 +259,23 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
 
   for (;;) {
     if (__ballot(want > 0) == 0) break;
+    // Issue priority by progress: the SIMD's arbiter favours older
+    // wavefronts, so mergers of equal size sharing a SIMD would otherwise
+    // finish far apart (the last ones alone, unable to hide latency); the
+    // merger furthest behind gets the highest priority.
+    {
+      const int64_t done4 = (obase - o0) * 4;
+      const int lvl = done4 < total ? 3 : done4 < 2 * total ? 2 : done4 < 3 * total ? 1 : 0;
+      if (lvl != prio) {
+        prio = lvl;
+        switch (lvl) {
+          case 3: __builtin_amdgcn_s_setprio(3); break;
+          case 2: __builtin_amdgcn_s_setprio(2); break;
+          case 1: __builtin_amdgcn_s_setprio(1); break;
+          default: __builtin_amdgcn_s_setprio(0); break;
+        }
+      }
+    }
 
     // ---- frontier y = min over full windows of the last element, tag (elem, list)
     const bool full = live && want == F;
