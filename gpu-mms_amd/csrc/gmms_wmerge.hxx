@@ -262,10 +262,12 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
     // Issue priority by progress: the SIMD's arbiter favours older
     // wavefronts, so mergers of equal size sharing a SIMD would otherwise
     // finish far apart (the last ones alone, unable to hide latency); the
-    // merger furthest behind gets the highest priority.
+    // merger furthest behind gets the highest priority. The bands narrow
+    // towards the end (1/2, 5/16, 1/8, 1/16 of the output), where the order
+    // among equal priorities decides who finishes last.
     {
-      const int64_t done4 = (obase - o0) * 4;
-      const int lvl = done4 < total ? 3 : done4 < 2 * total ? 2 : done4 < 3 * total ? 1 : 0;
+      const int64_t d16 = (obase - o0) * 16;
+      const int lvl = d16 < 8 * total ? 3 : d16 < 13 * total ? 2 : d16 < 15 * total ? 1 : 0;
       if (lvl != prio) {
         prio = lvl;
         switch (lvl) {
