@@ -190,8 +190,9 @@ int launch_batch(gmms_server* S, Batch* b) {
   const size_t kb = key_bytes(dt), vb = val_bytes(dt);
   int64_t o = 0;
   for (auto& r : b->reqs) {
-    HIP_OK(hipMemcpyAsync(base + L.k0 + o * kb, r->keys, (size_t)r->n * kb, hipMemcpyHostToDevice, s));
-    if (vb) HIP_OK(hipMemcpyAsync(base + L.v0 + o * vb, r->vals, (size_t)r->n * vb, hipMemcpyHostToDevice, s));
+    // request buffers in host or device memory (unified addressing picks the copy)
+    HIP_OK(hipMemcpyAsync(base + L.k0 + o * kb, r->keys, (size_t)r->n * kb, hipMemcpyDefault, s));
+    if (vb) HIP_OK(hipMemcpyAsync(base + L.v0 + o * vb, r->vals, (size_t)r->n * vb, hipMemcpyDefault, s));
     o += r->n;
   }
   size_t tb = 0, toff = 0;
@@ -211,8 +212,8 @@ int launch_batch(gmms_server* S, Batch* b) {
   const size_t rk = alt ? L.k1 : L.k0, rv = alt ? L.v1 : L.v0;
   o = 0;
   for (auto& r : b->reqs) {
-    HIP_OK(hipMemcpyAsync(r->out_keys, base + rk + o * kb, (size_t)r->n * kb, hipMemcpyDeviceToHost, s));
-    if (vb) HIP_OK(hipMemcpyAsync(r->out_vals, base + rv + o * vb, (size_t)r->n * vb, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(r->out_keys, base + rk + o * kb, (size_t)r->n * kb, hipMemcpyDefault, s));
+    if (vb) HIP_OK(hipMemcpyAsync(r->out_vals, base + rv + o * vb, (size_t)r->n * vb, hipMemcpyDefault, s));
     o += r->n;
   }
   HIP_OK(hipLaunchHostFunc(s, batch_done_cb, b));

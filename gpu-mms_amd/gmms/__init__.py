@@ -667,17 +667,23 @@ class Server:
         self.config = cfg
 
     def submit(self, keys, values=None, *, dtype: Optional[int] = None) -> Ticket:
-        """Queue a sort of host tensor `keys` (and `values` for pairs)."""
+        """Queue a sort of tensor `keys` (and `values` for pairs): host
+        tensors (results in pinned host tensors) or tensors on the server's
+        device (results on that device)."""
         torch = _torch()
-        if keys.is_cuda or (values is not None and values.is_cuda):
-            raise GmmsError(ERROR_INVALID_VALUE, "server requests are host tensors")
+        if values is not None and values.is_cuda != keys.is_cuda:
+            raise GmmsError(ERROR_INVALID_VALUE, "keys and values must live in the same memory")
         keys = keys.contiguous()
         dt = dtype if dtype is not None else (PAIR_U64_U32 if values is not None else dtype_of(keys))
-        out_k = torch.empty(keys.shape, dtype=keys.dtype, pin_memory=True)
+        if keys.is_cuda:
+            out_k = torch.empty_like(keys)
+        else:
+            out_k = torch.empty(keys.shape, dtype=keys.dtype, pin_memory=True)
         out_v = None
         if values is not None:
             values = values.contiguous()
-            out_v = torch.empty(values.shape, dtype=values.dtype, pin_memory=True)
+            out_v = torch.empty_like(values) if values.is_cuda else torch.empty(values.shape, dtype=values.dtype,
+                                                                                 pin_memory=True)
         tid = ctypes.c_uint64(0)
         st = lib().gmms_server_submit(self._h, dt, keys.data_ptr(), _ptr(values), keys.numel(), out_k.data_ptr(),
                                       _ptr(out_v), ctypes.byref(tid))

@@ -313,8 +313,9 @@ int gmms_server_default_config(gmms_server_config* cfg);
 int gmms_server_request_bytes(int dtype, int64_t n, size_t* bytes);
 int gmms_server_create(const gmms_server_config* cfg, gmms_server** server);
 /* Queue a request: sort n elements of keys (and vals for pairs) into
- * out_keys (out_vals); input and output buffers must stay valid until the
- * request completes. *ticket identifies it. Returns at once. */
+ * out_keys (out_vals); the buffers may be device memory or (pinned) host
+ * memory, and must stay valid until the request completes. *ticket
+ * identifies it. Returns at once. */
 int gmms_server_submit(gmms_server* server, int dtype, const void* keys, const void* vals, int64_t n,
                        void* out_keys, void* out_vals, uint64_t* ticket);
 /* Wait up to timeout_s seconds (< 0: forever) for a request; *done = 1 once

@@ -73,6 +73,41 @@ def test_results_same_alone_and_in_mix(gmms, oracle, dev):
             assert np.array_equal(_bits(m), _bits(a))
 
 
+def test_device_resident_requests(gmms, oracle, dev):
+    """Requests whose buffers live on the device (copied device to device
+    into the arena and out of it) give the same results as host requests,
+    alone and in a mix with host requests."""
+    import torch
+
+    print("server: device-resident requests", flush=True)
+    reqs = _requests(oracle)
+
+    def to_dev(kind, k, v):
+        if kind == "pair":
+            return _pinned(k.view(np.int64)).to(dev), _pinned(v.view(np.int32)).to(dev)
+        return _pinned(k).to(dev), None
+
+    with gmms.Server(hbm_budget=1 << 30) as srv:
+        ts = []
+        for i, (kind, k, v) in enumerate(reqs):
+            if i % 2:
+                ts.append(_submit(srv, kind, k, v))
+            else:
+                dk, dv = to_dev(kind, k, v)
+                ts.append(srv.submit(dk, dv, dtype=gmms.PAIR_U64_U32) if kind == "pair" else srv.submit(dk))
+        outs = [srv.result(t) for t in ts]
+        assert srv.stats().failed == 0
+    for i, ((kind, k, v), o) in enumerate(zip(reqs, outs)):
+        if kind == "pair":
+            assert (o[0].is_cuda and o[1].is_cuda) == (i % 2 == 0)
+            wk, wv = oracle.sort_pairs(k, v)
+            assert np.array_equal(o[0].cpu().numpy().view(np.uint64), wk)
+            assert np.array_equal(o[1].cpu().numpy().view(np.uint32), wv)
+        else:
+            assert o.is_cuda == (i % 2 == 0)
+            assert np.array_equal(_bits(o.cpu().numpy()), _bits(oracle.sort(k)))
+
+
 def test_budget_bounds_memory_and_batches_overlap(gmms, oracle, dev):
     print("server: 48 MB budget, 40 requests", flush=True)
     budget = 48 << 20
