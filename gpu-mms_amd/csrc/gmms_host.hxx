@@ -136,7 +136,20 @@ inline int mergers_per_task(int64_t n, int64_t L, int K, int target) {
   const int64_t TK = (int64_t)K * L;
   const int64_t tasks = (n + TK - 1) / TK;
   if (tasks >= target) return 1;
+  // target = the mergers resident at once. A merger past a multiple of it
+  // waits for a free slot and then runs a whole merger's time at the end of
+  // the round, so fill whole generations: the smallest m with
+  // tasks*m = k*target (k = 1..4) to within 10 %, else ceil(target/tasks).
   int64_t m = (target + tasks - 1) / tasks;
+  for (int k = 1; k <= 4; ++k) {
+    const int64_t mk = (int64_t)k * target / tasks;
+    if (mk < 1) continue;
+    const int64_t used = tasks * mk, slots = (used + target - 1) / target * target;
+    if (used * 10 >= slots * 9) {
+      m = mk;
+      break;
+    }
+  }
   // keep every merger at least a few heap-builds worth of output
   const int64_t minsz = (int64_t)WAVE * K * 4;
   const int64_t tsz = std::min<int64_t>(n, TK);
