@@ -147,6 +147,24 @@ def test_pairs(gmms, oracle, dev, dist):
     assert np.array_equal(_host(vs).view(np.uint32), wv)
 
 
+@pytest.mark.parametrize("n,mergers", [(600_001, 37), (3_000_017, 101), (3_000_017, 3072)])
+def test_pairs_split_tasks(gmms, oracle, dev, n, mergers):
+    """Tasks split over several mergers each (the merger count fills whole
+    generations of `mergers` resident mergers, so a task of 600_001 over 37
+    gets 7 mergers), on duplicate keys whose ties are broken by value."""
+    import torch
+
+    print(f"pairs n={n} mergers={mergers}", flush=True)
+    k, v = oracle.gen_pairs(31 + mergers, n, 0)
+    k = k >> np.uint64(48)
+    p = gmms.make_params(K=16, mergers=mergers)
+    ks, vs = gmms.sort(torch.from_numpy(k.view(np.int64)).to(dev), torch.from_numpy(v.view(np.int32)).to(dev),
+                       params=p)
+    wk, wv = oracle.sort_pairs(k, v)
+    assert np.array_equal(_host(ks).view(np.uint64), wk)
+    assert np.array_equal(_host(vs).view(np.uint32), wv)
+
+
 def test_find_partitions_cuts_are_consistent(gmms, oracle, dev):
     n, L, K = 1 << 18, 1 << 14, 16
     print("find_partitions", flush=True)
