@@ -246,6 +246,25 @@ __device__ __forceinline__ void tile_store(void* keys, uint32_t* vals, int64_t b
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (cnt == TILE) {
+      // lane l stores the four consecutive ranks 256u + 4l .. + 3 of each
+      // row u: two 16-byte key stores (and one 16-byte value store)
+#pragma unroll
+      for (int u = 0; u < TPL / 4; ++u) {
+        const int o = u * 256 + 4 * lane;
+        E e4[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int slot = (o / TPL) * XPOSE_STRIDE + (o % TPL) + t;  // o % TPL + t < TPL
+          uint32_t w[E::W];
+#pragma unroll
+          for (int q = 0; q < E::W; ++q) w[q] = my[q * XPOSE_WORDS + slot];
+          e4[t] = from_words<E>(w);
+        }
+        gstore4<DT, CONV>(keys, vals, base, o, e4, flip);
+      }
+      return;
+    }
 #pragma unroll
     for (int r = 0; r < TPL; ++r) {
       const int e = r * WAVE + lane;  // rank
