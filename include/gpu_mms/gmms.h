@@ -65,7 +65,10 @@ typedef struct gmms_params {
                         wavefront and heap mergers; for the workgroup tile
                         merger the largest power of two <= 64 that fits LDS) */
   int32_t pipeline;  /* merge kernel, GMMS_MERGE_*                           */
-  int32_t mergers;   /* target mergers per round when tasks are few; 0: auto */
+  int32_t mergers;   /* mergers that run at once when tasks are few: a
+                        round splits its tasks into whole multiples of it
+                        (up to 4x, to within 10 %); 0: auto, the resident
+                        count of the merge kernel                          */
   int32_t prefetch;  /* blocks prefetched per list into LDS (1..8)           */
   int32_t slack;     /* partition tolerance in elements; 0: auto             */
   int32_t base_run;  /* sorted run length the base case leaves: 1024 (one

@@ -11,9 +11,11 @@
  *             merger per wavefront (default, fastest), 1 the pipelined
  *             minBlockHeap, 2 tile merge with one merger per workgroup
  *   RANGE     range of the harness's random keys, rand() % RANGE
- *   BLOCKS    mergers per round when a round has few tasks (reference: 482;
- *             here 0 = auto: the resident merger count of the merge kernel,
- *             12 per compute unit for the default one, 3072 on MI355X)
+ *   BLOCKS    mergers that run at once when a round has few tasks
+ *             (reference: 482); a round splits its tasks into a whole
+ *             multiple of it (up to 4x, filled to within 10 %). 0 = auto:
+ *             the resident merger count of the merge kernel, for the default
+ *             one 16 per compute unit (4096 on MI355X), 12 for pairs
  *   M         base-case tile (1024 elements, one wavefront)
  *   W         wavefront width on CDNA4
  *   B         elements per heap node (one per lane)
