@@ -260,6 +260,9 @@ inline int dispatch(int dtype, F&& f) {
 // (the caller has copied the table block to ws + *ws_offset on the stream).
 const void* seg_plan_table(const gmms_seg_plan* plan, size_t* bytes, size_t* ws_offset);
 size_t seg_plan_workspace(const gmms_seg_plan* plan);
+// The workspace bytes of the plan of these segments (default parameters),
+// from its counts alone: no tables are built.
+int seg_plan_workspace_for(int dtype, const int64_t* offsets, int64_t n_seg, size_t* bytes);
 int seg_plan_launch(const gmms_seg_plan* plan, void* keys, void* keys_alt, void* vals, void* vals_alt, void* ws,
                     hipStream_t s, int* alt);
 

@@ -128,7 +128,9 @@ struct gmms_seg_graph {
 
 namespace {
 
-int plan_build(gmms_seg_plan* P, int dtype, const int64_t* offsets, int64_t n_seg, const gmms_params* p) {
+// sizing_only: counts, offsets and the workspace size, without the tables.
+int plan_build(gmms_seg_plan* P, int dtype, const int64_t* offsets, int64_t n_seg, const gmms_params* p,
+               bool sizing_only = false) {
   Resolved r;
   int st = resolve(dtype, p, &r, true);
   if (st) return st;
@@ -178,7 +180,7 @@ int plan_build(gmms_seg_plan* P, int dtype, const int64_t* offsets, int64_t n_se
   // segment table: the 4096-element run descriptors of all long segments
   // come first, then the tile descriptors of the short ones
   std::vector<SegDesc> segs;
-  segs.reserve(nonempty);
+  if (!sizing_only) segs.reserve(nonempty);
   int64_t r4 = 0, r1 = 0;
   for (int64_t s = 0; s < n_seg; ++s) {
     const int64_t n = offsets[s + 1] - offsets[s];
@@ -188,7 +190,7 @@ int plan_build(gmms_seg_plan* P, int dtype, const int64_t* offsets, int64_t n_se
   P->runs4 = r4;
   P->runs1 = r1;
   int64_t a4 = 0, a1 = r4;
-  for (int64_t s = 0; s < n_seg; ++s) {
+  for (int64_t s = 0; s < (sizing_only ? 0 : n_seg); ++s) {
     const int64_t n = offsets[s + 1] - offsets[s];
     if (n == 0) continue;
     SegDesc d{};
@@ -213,75 +215,90 @@ int plan_build(gmms_seg_plan* P, int dtype, const int64_t* offsets, int64_t n_se
     const int64_t n = offsets[s + 1] - offsets[s];
     for (int g = R - P->seg_R[s]; g < R; ++g) vol[g] += n;
   }
-  std::vector<std::vector<MergerInfo>> merge_items(R), part_items(R);
-  std::vector<int64_t> rows(R, 0);
-  for (int64_t s = 0; s < n_seg; ++s) {
-    const int64_t n = offsets[s + 1] - offsets[s];
-    const int rs = P->seg_R[s];
-    const int K = P->seg_K[s];
-    int64_t L = n > TILE ? (int64_t)TILE * BASE_WAVES : TILE;
-    for (int rr = 0; rr < rs; ++rr, L *= K) {
-      const int g = R - rs + rr;
-      const int64_t TK = (int64_t)K * L;
-      const int64_t tasks = (n + TK - 1) / TK;
-      const int64_t minsz = (int64_t)WAVE * K * 4;
-      const int64_t target =
-          seg_merger[s] > 0 ? seg_merger[s] : std::max<int64_t>(minsz, (vol[g] + P->slots - 1) / P->slots);
-      for (int64_t t = 0; t < tasks; ++t) {
-        const int64_t tb = t * TK;
-        const int64_t T = std::min(TK, n - tb);
-        int64_t m = (T + target - 1) / target;
-        m = std::min<int64_t>(m, std::max<int64_t>(1, T / minsz));
-        m = std::max<int64_t>(1, m);
-        MergerInfo mi{};
-        mi.tbase = offsets[s] + tb;
-        mi.T = T;
-        mi.L = L;
-        mi.cut_row = m > 1 ? rows[g] : 0;
-        mi.m = (int32_t)m;
-        mi.K = K;
-        if (m > 1) rows[g] += m;
-        for (int64_t j = 0; j < m; ++j) {
-          mi.j = (int32_t)j;
-          merge_items[g].push_back(mi);
-          if (j > 0) part_items[g].push_back(mi);
+  // Every task of every segment's rounds, in (segment, round, task) order:
+  // emit(g, s, tbase, T, L, K, m) with the task's merger count m.
+  auto walk = [&](auto&& emit) {
+    for (int64_t s = 0; s < n_seg; ++s) {
+      const int64_t n = offsets[s + 1] - offsets[s];
+      const int rs = P->seg_R[s];
+      const int K = P->seg_K[s];
+      int64_t L = n > TILE ? (int64_t)TILE * BASE_WAVES : TILE;
+      for (int rr = 0; rr < rs; ++rr, L *= K) {
+        const int g = R - rs + rr;
+        const int64_t TK = (int64_t)K * L;
+        const int64_t tasks = (n + TK - 1) / TK;
+        const int64_t minsz = (int64_t)WAVE * K * 4;
+        const int64_t target =
+            seg_merger[s] > 0 ? seg_merger[s] : std::max<int64_t>(minsz, (vol[g] + P->slots - 1) / P->slots);
+        for (int64_t t = 0; t < tasks; ++t) {
+          const int64_t tb = t * TK;
+          const int64_t T = std::min(TK, n - tb);
+          int64_t m = (T + target - 1) / target;
+          m = std::min<int64_t>(m, std::max<int64_t>(1, T / minsz));
+          emit(g, s, tb, T, L, K, std::max<int64_t>(1, m));
         }
       }
     }
-  }
-  // LPT order inside each round: larger mergers first (stable for equal sizes)
-  for (int g = 0; g < R; ++g) {
-    std::stable_sort(merge_items[g].begin(), merge_items[g].end(), [](const MergerInfo& x, const MergerInfo& y) {
-      return x.T / x.m > y.T / y.m;
-    });
-  }
+  };
+  // pass 1: item and cut-row counts per round (all that sizing needs)
+  std::vector<int64_t> n_items(R, 0), n_part(R, 0), rows(R, 0);
+  walk([&](int g, int64_t, int64_t, int64_t, int64_t, int, int64_t m) {
+    n_items[g] += m;
+    n_part[g] += m - 1;
+    if (m > 1) rows[g] += m;
+  });
   P->rounds.assign(R, SegRound{});
   int64_t off = 0;
   for (int g = 0; g < R; ++g) {
     SegRound& sr = P->rounds[g];
     sr.items_off = off;
-    sr.n_items = (int64_t)merge_items[g].size();
+    sr.n_items = n_items[g];
     off += sr.n_items;
     sr.part_off = off;
-    sr.n_part = (int64_t)part_items[g].size();
+    sr.n_part = n_part[g];
     off += sr.n_part;
     sr.elems = vol[g];
     sr.rows = rows[g];
     P->max_rows = std::max(P->max_rows, rows[g]);
   }
   P->n_items_all = off;
-
-  // staging block (host; copied to pinned memory on the first sort)
-  P->items_byte_off = align256(segs.size() * sizeof(SegDesc));
+  P->items_byte_off = align256((size_t)nonempty * sizeof(SegDesc));
   P->host_bytes = P->items_byte_off + (size_t)off * sizeof(MergerInfo);
-  P->table.assign(P->host_bytes, 0);
-  if (P->host_bytes) {
+
+  // pass 2: the staging block (host; copied to pinned memory on the first
+  // sort): segment table, then per round its merger items and partition items
+  if (!sizing_only) {
+    P->table.assign(P->host_bytes, 0);
     char* h = P->table.data();
     if (!segs.empty()) std::memcpy(h, segs.data(), segs.size() * sizeof(SegDesc));
     MergerInfo* it = (MergerInfo*)(h + P->items_byte_off);
+    std::vector<int64_t> mi_at(R), pi_at(R), row_at(R, 0);
     for (int g = 0; g < R; ++g) {
-      std::copy(merge_items[g].begin(), merge_items[g].end(), it + P->rounds[g].items_off);
-      std::copy(part_items[g].begin(), part_items[g].end(), it + P->rounds[g].part_off);
+      mi_at[g] = P->rounds[g].items_off;
+      pi_at[g] = P->rounds[g].part_off;
+    }
+    walk([&](int g, int64_t s, int64_t tb, int64_t T, int64_t L, int K, int64_t m) {
+      MergerInfo mi{};
+      mi.tbase = offsets[s] + tb;
+      mi.T = T;
+      mi.L = L;
+      mi.cut_row = m > 1 ? row_at[g] : 0;
+      mi.m = (int32_t)m;
+      mi.K = K;
+      if (m > 1) row_at[g] += m;
+      for (int64_t j = 0; j < m; ++j) {
+        mi.j = (int32_t)j;
+        it[mi_at[g]++] = mi;
+        if (j > 0) it[pi_at[g]++] = mi;
+      }
+    });
+    // LPT order inside each round: larger mergers first (stable for equal
+    // sizes; a round of one segment's equal tasks is in that order already)
+    const auto larger = [](const MergerInfo& x, const MergerInfo& y) { return x.T / x.m > y.T / y.m; };
+    for (int g = 0; g < R; ++g) {
+      MergerInfo* b = it + P->rounds[g].items_off;
+      MergerInfo* e = b + P->rounds[g].n_items;
+      if (!std::is_sorted(b, e, larger)) std::stable_sort(b, e, larger);
     }
   }
   // workspace: queue counters | table block | tiles | cuts
@@ -443,6 +460,13 @@ const void* seg_plan_table(const gmms_seg_plan* P, size_t* bytes, size_t* ws_off
 }
 
 size_t seg_plan_workspace(const gmms_seg_plan* P) { return P->ws_bytes; }
+
+int seg_plan_workspace_for(int dtype, const int64_t* offsets, int64_t n_seg, size_t* bytes) {
+  gmms_seg_plan P;
+  const int st = plan_build(&P, dtype, offsets, n_seg, nullptr, true);
+  *bytes = st ? 0 : P.ws_bytes;
+  return st;
+}
 
 int seg_plan_launch(const gmms_seg_plan* P, void* keys, void* keys_alt, void* vals, void* vals_alt, void* ws,
                     hipStream_t s, int* alt) {

@@ -227,15 +227,17 @@ gmms_seg_plan* fit_batch(gmms_server* S, std::vector<std::shared_ptr<Request>>& 
   while (!reqs.empty()) {
     std::vector<int64_t> offs(reqs.size() + 1, 0);
     for (size_t i = 0; i < reqs.size(); ++i) offs[i + 1] = offs[i] + reqs[i]->n;
-    gmms_seg_plan* P = nullptr;
-    if (gmms_seg_plan_create(reqs[0]->dtype, offs.data(), (int64_t)reqs.size(), nullptr, &P) != GMMS_SUCCESS)
+    size_t ws = 0;  // sized from the plan's counts; the tables are built once it fits
+    if (seg_plan_workspace_for(reqs[0]->dtype, offs.data(), (int64_t)reqs.size(), &ws) != GMMS_SUCCESS)
       return nullptr;
-    const Layout L = layout_for(reqs[0]->dtype, offs.back(), seg_plan_workspace(P));
+    const Layout L = layout_for(reqs[0]->dtype, offs.back(), ws);
     if (L.total <= free_bytes) {
-      *bytes = L.total;
+      gmms_seg_plan* P = nullptr;
+      if (gmms_seg_plan_create(reqs[0]->dtype, offs.data(), (int64_t)reqs.size(), nullptr, &P) != GMMS_SUCCESS)
+        return nullptr;
+      *bytes = layout_for(reqs[0]->dtype, offs.back(), seg_plan_workspace(P)).total;
       return P;
     }
-    gmms_seg_plan_destroy(P);
     reqs.pop_back();
   }
   return nullptr;
@@ -467,11 +469,10 @@ int gmms_server_submit(gmms_server* S, int dtype, const void* keys, const void* 
   if (n > 0) {
     // a request alone, with its tables, must fit the budget
     int64_t offs[2] = {0, n};
-    gmms_seg_plan* P = nullptr;
-    int st = gmms_seg_plan_create(dtype, offs, 1, nullptr, &P);
+    size_t ws = 0;
+    int st = seg_plan_workspace_for(dtype, offs, 1, &ws);
     if (st) return st;
-    const size_t need = layout_for(dtype, n, seg_plan_workspace(P)).total;
-    gmms_seg_plan_destroy(P);
+    const size_t need = layout_for(dtype, n, ws).total;
     if (need > S->cfg.hbm_budget) {
       std::lock_guard<std::mutex> g(S->mu);
       S->st.rejected++;
