@@ -256,20 +256,30 @@ def serving(args, oracle, threads):
     torch.cuda.synchronize()
     srv = gmms.Server()  # 64 GB budget
 
-    def submit(k, v):
-        return srv.submit(k) if v is None else srv.submit(k, v, dtype=gmms.PAIR_U64_U32)
+    def submit(k, v, outs=(None, None)):
+        if v is None:
+            return srv.submit(k, out_keys=outs[0])
+        return srv.submit(k, v, dtype=gmms.PAIR_U64_U32, out_keys=outs[0], out_values=outs[1])
 
     # saturation: every request submitted at once (closed batch), for the
-    # server's peak rate on this mix
+    # server's peak rate on this mix; the callers' result buffers exist
+    # beforehand, as in a serving loop that reuses them
+    outs = [(torch.empty_like(k), None if v is None else torch.empty_like(v)) for k, v in dreqs]
+    # one untimed pass first: the first launches of each dtype's kernels
+    # load their code objects
+    for t in [submit(k, v, o) for (k, v), o in zip(dreqs, outs)]:
+        srv.wait(t)
+    torch.cuda.synchronize()
+    b0 = srv.stats().batches
     ts0 = time.perf_counter()
-    sat = [submit(k, v) for k, v in dreqs]
+    sat = [submit(k, v, o) for (k, v), o in zip(dreqs, outs)]
     srv.drain()
     sat_wall = time.perf_counter() - ts0
     for t in sat:
         srv.wait(t)
     sat_lat = np.array([t.latency_ms for t in sat])
     sat_ok = all(t.status == 0 for t in sat)
-    sat_batches = srv.stats().batches
+    sat_batches = srv.stats().batches - b0
     log(f"serving: saturated run {sat_wall:.3f} s")
 
     def poisson(rs):
@@ -347,7 +357,7 @@ def serving(args, oracle, threads):
                       "p50_ms": round(float(np.percentile(sat_lat, 50)), 3),
                       "p99_ms": round(float(np.percentile(sat_lat, 99)), 3),
                       "batches": sat_batches,
-                      "note": "the same device-resident requests submitted at once; results checked too"},
+                      "note": "the same device-resident requests submitted at once into result buffers allocated beforehand; results checked too"},
         "host_buffers": {"p50_ms": round(float(np.percentile(hlat, 50)), 3),
                          "p99_ms": round(float(np.percentile(hlat, 99)), 3),
                          "max_ms": round(float(hlat.max()), 3),

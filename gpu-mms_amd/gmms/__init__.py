@@ -666,24 +666,34 @@ class Server:
         self._h = h
         self.config = cfg
 
-    def submit(self, keys, values=None, *, dtype: Optional[int] = None) -> Ticket:
+    def submit(self, keys, values=None, *, dtype: Optional[int] = None, out_keys=None,
+               out_values=None) -> Ticket:
         """Queue a sort of tensor `keys` (and `values` for pairs): host
         tensors (results in pinned host tensors) or tensors on the server's
-        device (results on that device)."""
+        device (results on that device). `out_keys` / `out_values` receive
+        the results when given (same shape, dtype and memory as the inputs,
+        contiguous); otherwise they are allocated here."""
         torch = _torch()
         if values is not None and values.is_cuda != keys.is_cuda:
             raise GmmsError(ERROR_INVALID_VALUE, "keys and values must live in the same memory")
         keys = keys.contiguous()
         dt = dtype if dtype is not None else (PAIR_U64_U32 if values is not None else dtype_of(keys))
-        if keys.is_cuda:
-            out_k = torch.empty_like(keys)
-        else:
-            out_k = torch.empty(keys.shape, dtype=keys.dtype, pin_memory=True)
+
+        def out_like(x, given):
+            if given is not None:
+                if (given.shape != x.shape or given.dtype != x.dtype or given.is_cuda != x.is_cuda
+                        or not given.is_contiguous()):
+                    raise GmmsError(ERROR_INVALID_VALUE, "output tensors must match their inputs")
+                return given
+            if x.is_cuda:
+                return torch.empty_like(x)
+            return torch.empty(x.shape, dtype=x.dtype, pin_memory=True)
+
+        out_k = out_like(keys, out_keys)
         out_v = None
         if values is not None:
             values = values.contiguous()
-            out_v = torch.empty_like(values) if values.is_cuda else torch.empty(values.shape, dtype=values.dtype,
-                                                                                 pin_memory=True)
+            out_v = out_like(values, out_values)
         tid = ctypes.c_uint64(0)
         st = lib().gmms_server_submit(self._h, dt, keys.data_ptr(), _ptr(values), keys.numel(), out_k.data_ptr(),
                                       _ptr(out_v), ctypes.byref(tid))

@@ -108,6 +108,45 @@ def test_device_resident_requests(gmms, oracle, dev):
             assert np.array_equal(_bits(o.cpu().numpy()), _bits(oracle.sort(k)))
 
 
+def test_caller_result_buffers(gmms, oracle, dev):
+    """Results land in the output tensors the caller passes (device and
+    pinned host); a mismatched output is refused before anything is queued."""
+    import torch
+
+    print("server: caller result buffers", flush=True)
+    reqs = _requests(oracle)
+    with gmms.Server(hbm_budget=1 << 30) as srv:
+        ts, given = [], []
+        for i, (kind, k, v) in enumerate(reqs):
+            if kind == "pair":
+                ik, iv = _pinned(k.view(np.int64)), _pinned(v.view(np.int32))
+            else:
+                ik, iv = _pinned(k), None
+            if i % 2 == 0:
+                ik = ik.to(dev)
+                iv = None if iv is None else iv.to(dev)
+            ok_ = torch.empty_like(ik) if ik.is_cuda else torch.empty(ik.shape, dtype=ik.dtype, pin_memory=True)
+            ov_ = None
+            if iv is not None:
+                ov_ = torch.empty_like(iv) if iv.is_cuda else torch.empty(iv.shape, dtype=iv.dtype, pin_memory=True)
+            dt = gmms.PAIR_U64_U32 if kind == "pair" else None
+            ts.append(srv.submit(ik, iv, dtype=dt, out_keys=ok_, out_values=ov_))
+            given.append((ok_, ov_))
+        with pytest.raises(gmms.GmmsError):
+            srv.submit(_pinned(reqs[0][1]), out_keys=torch.empty(3, dtype=torch.int64))
+        outs = [srv.result(t) for t in ts]
+        assert srv.stats().failed == 0
+    for (kind, k, v), o, (gk, gv) in zip(reqs, outs, given):
+        if kind == "pair":
+            assert o[0] is gk and o[1] is gv
+            wk, wv = oracle.sort_pairs(k, v)
+            assert np.array_equal(gk.cpu().numpy().view(np.uint64), wk)
+            assert np.array_equal(gv.cpu().numpy().view(np.uint32), wv)
+        else:
+            assert o is gk
+            assert np.array_equal(_bits(gk.cpu().numpy()), _bits(oracle.sort(k)))
+
+
 def test_budget_bounds_memory_and_batches_overlap(gmms, oracle, dev):
     print("server: 48 MB budget, 40 requests", flush=True)
     budget = 48 << 20
