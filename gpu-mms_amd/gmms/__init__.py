@@ -672,7 +672,8 @@ class Server:
         tensors (results in pinned host tensors) or tensors on the server's
         device (results on that device). `out_keys` / `out_values` receive
         the results when given (same shape, dtype and memory as the inputs,
-        contiguous); otherwise they are allocated here."""
+        contiguous, pinned when on the host); otherwise they are allocated
+        here."""
         torch = _torch()
         if values is not None and values.is_cuda != keys.is_cuda:
             raise GmmsError(ERROR_INVALID_VALUE, "keys and values must live in the same memory")
@@ -682,8 +683,9 @@ class Server:
         def out_like(x, given):
             if given is not None:
                 if (given.shape != x.shape or given.dtype != x.dtype or given.is_cuda != x.is_cuda
-                        or not given.is_contiguous()):
-                    raise GmmsError(ERROR_INVALID_VALUE, "output tensors must match their inputs")
+                        or not given.is_contiguous() or (not given.is_cuda and not given.is_pinned())):
+                    raise GmmsError(ERROR_INVALID_VALUE, "output tensors must match their inputs "
+                                    "(host outputs in pinned memory)")
                 return given
             if x.is_cuda:
                 return torch.empty_like(x)
