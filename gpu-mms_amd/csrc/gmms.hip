@@ -250,6 +250,32 @@ __global__ void k_pad(void* keys, uint32_t* vals, int64_t count) {
   if (i < count) gstore<DT, true>(keys, vals, i, Limits<E>::hi());
 }
 
+// Workspace of gmms_sort_noalloc: one array per device for the life of the
+// process, so that call allocates nothing. Host code must never pass the
+// array's name to a kernel (in host code it is a stand-in, not a device
+// address): scratch_ptr() returns this device's copy, looked up once per
+// device with hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[GMMS_NOALLOC_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+int scratch_ptr(void** out) {
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return GMMS_ERROR_UNSUPPORTED;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return GMMS_ERROR_HIP;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return GMMS_SUCCESS;
+}
+
 }  // namespace
 
 extern "C" {
@@ -398,6 +424,37 @@ int gmms_sort(int dtype, void* keys, void* keys_alt, void* vals, void* vals_alt,
   return dispatch(dtype, [&](auto dt) {
     return run_sort<decltype(dt)::value>(keys, keys_alt, vals, vals_alt, n, r, ws, wsb, s, alt);
   });
+}
+
+int gmms_noalloc_scratch_bytes(size_t* bytes) {
+  if (!bytes) return GMMS_ERROR_INVALID_VALUE;
+  *bytes = GMMS_NOALLOC_SCRATCH_BYTES;
+  return GMMS_SUCCESS;
+}
+
+int gmms_sort_noalloc(int dtype, void* keys, void* keys_alt, void* vals, void* vals_alt, int64_t n,
+                      const gmms_params* p, void* stream, int* alt) {
+  if (!valid_dtype(dtype) || n < 0) return GMMS_ERROR_INVALID_VALUE;
+  if (alt) *alt = 0;
+  if (n == 0) return GMMS_SUCCESS;
+  Resolved r;
+  int st = resolve(dtype, p, &r, true);
+  if (st) return st;
+  const size_t need = ws_bytes_for(n, r);
+  if (need > GMMS_NOALLOC_SCRATCH_BYTES) {
+    std::snprintf(g_last_error, sizeof(g_last_error),
+                  "gmms_sort_noalloc: n=%lld needs %zu workspace bytes, the static scratch holds %zu",
+                  (long long)n, need, (size_t)GMMS_NOALLOC_SCRATCH_BYTES);
+    return GMMS_ERROR_WORKSPACE;
+  }
+  void* ws = nullptr;
+  if (need) {
+    st = scratch_ptr(&ws);
+    if (st) return st;
+  }
+  // The sort writes every workspace word it reads (samples, then cuts) in
+  // the same call, so the scratch needs no clearing between calls.
+  return gmms_sort(dtype, keys, keys_alt, vals, vals_alt, n, p, ws, need, stream, alt);
 }
 
 int gmms_multimergesort_params(int dtype, void* input, void* output, void* host_array, const gmms_params* pp,

@@ -155,6 +155,8 @@ def lib() -> ctypes.CDLL:
         "gmms_round_workspace_size": (i32, [i32, i64, i64, P, ctypes.POINTER(sz)]),
         "gmms_sort": (i32, [i32, vp, vp, vp, vp, i64, P, vp, sz, vp, ctypes.POINTER(i32)]),
         "gmms_multimergesort": (i32, [i32, vp, vp, vp, i32, i64, ctypes.POINTER(vp)]),
+        "gmms_sort_noalloc": (i32, [i32, vp, vp, vp, vp, i64, P, vp, ctypes.POINTER(i32)]),
+        "gmms_noalloc_scratch_bytes": (i32, [ctypes.POINTER(sz)]),
         "gmms_squaresort": (i32, [i32, vp, vp, i64, i32, i32, vp]),
         "gmms_merge_round": (i32, [i32, vp, vp, vp, vp, i64, i64, P, i32, vp, sz, vp]),
         "gmms_find_partitions": (i32, [i32, vp, vp, i64, i64, P, vp, vp, i64, ctypes.POINTER(i32), vp]),

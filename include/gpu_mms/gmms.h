@@ -124,6 +124,23 @@ int gmms_sort(int dtype, void* keys, void* keys_alt, void* vals, void* vals_alt,
               int64_t n, const gmms_params* p, void* workspace,
               size_t workspace_bytes, void* stream, int* result_in_alt);
 
+/* gmms_sort without a workspace argument and without any allocation. No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The workspace is a static device array
+ * of GMMS_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; a
+ * sort whose workspace (gmms_workspace_size) exceeds it returns
+ * GMMS_ERROR_WORKSPACE (16 MiB with the default knobs on 256 compute units:
+ * 32-bit keys up to about 1.03e9 elements, 64-bit keys about 5.1e8, pairs
+ * about 3.4e8). Every call on one GPU shares
+ * that array, so calls that overlap on one GPU (two streams, two threads)
+ * race; serialise them. Otherwise as gmms_sort. */
+#define GMMS_NOALLOC_SCRATCH_BYTES ((size_t)16 << 20)
+int gmms_sort_noalloc(int dtype, void* keys, void* keys_alt, void* vals, void* vals_alt,
+                      int64_t n, const gmms_params* p, void* stream, int* result_in_alt);
+/* The static scratch size in bytes (GMMS_NOALLOC_SCRATCH_BYTES). */
+int gmms_noalloc_scratch_bytes(size_t* bytes);
+
 /* The reference's entry point with its own argument list
  * (src/multimergesort.hxx:56-57 plus README.md's threadBlocks): sorts the
  * device array `input` of n elements using `output` as the second buffer,
