@@ -93,6 +93,10 @@ struct Request {
   int status = GMMS_SUCCESS;
   bool done = false;
   int64_t batch = -1;
+  // recorded on the caller's stream at submission (gmms_server_submit_after):
+  // the batch's copies wait for it, so they follow the work that produces
+  // the inputs and that last used the result buffers
+  hipEvent_t after = nullptr;
 };
 
 struct Batch;
@@ -189,6 +193,8 @@ int launch_batch(gmms_server* S, Batch* b) {
   char* base = (char*)S->arena_dev + b->off;
   const size_t kb = key_bytes(dt), vb = val_bytes(dt);
   int64_t o = 0;
+  for (auto& r : b->reqs)
+    if (r->after) HIP_OK(hipStreamWaitEvent(s, r->after, 0));
   for (auto& r : b->reqs) {
     // request buffers in host or device memory (unified addressing picks the copy)
     HIP_OK(hipMemcpyAsync(base + L.k0 + o * kb, r->keys, (size_t)r->n * kb, hipMemcpyDefault, s));
@@ -268,11 +274,17 @@ void scheduler(gmms_server* S) {
   for (;;) {
     // 1. retire finished batches
     std::vector<gmms_seg_plan*> dead;
+    std::vector<hipEvent_t> dead_ev;
     for (size_t i = 0; i < S->inflight.size();) {
       Batch* b = S->inflight[i].get();
       if (b->done) {
         finish(S, b);
         dead.push_back(b->plan);
+        for (auto& r : b->reqs)
+          if (r->after) {
+            dead_ev.push_back(r->after);
+            r->after = nullptr;
+          }
         S->inflight.erase(S->inflight.begin() + i);
       } else {
         ++i;
@@ -348,12 +360,14 @@ void scheduler(gmms_server* S) {
         (void)hipStreamSynchronize(S->streams[bp->slot]);
         lk.lock();
         bp->status = st;
+        bp->t_done = Clock::now();
         bp->done = true;
       }
     }
     if (!dead.empty()) {
       lk.unlock();
       for (gmms_seg_plan* P : dead) gmms_seg_plan_destroy(P);
+      for (hipEvent_t e : dead_ev) (void)hipEventDestroy(e);
       lk.lock();
       continue;  // re-check: the state may have changed meanwhile
     }
@@ -454,6 +468,11 @@ int gmms_server_create(const gmms_server_config* cfg, gmms_server** out) {
 
 int gmms_server_submit(gmms_server* S, int dtype, const void* keys, const void* vals, int64_t n, void* out_keys,
                        void* out_vals, uint64_t* ticket) {
+  return gmms_server_submit_after(S, dtype, keys, vals, n, out_keys, out_vals, nullptr, 0, ticket);
+}
+
+int gmms_server_submit_after(gmms_server* S, int dtype, const void* keys, const void* vals, int64_t n,
+                             void* out_keys, void* out_vals, void* stream, int ordered, uint64_t* ticket) {
   if (!S || !ticket || !valid_dtype(dtype) || n < 0) return GMMS_ERROR_INVALID_VALUE;
   if (n > 0 && (!keys || !out_keys || has_vals(dtype) != (vals != nullptr) ||
                 has_vals(dtype) != (out_vals != nullptr)))
@@ -479,8 +498,19 @@ int gmms_server_submit(gmms_server* S, int dtype, const void* keys, const void* 
       return GMMS_ERROR_BUDGET;
     }
   }
+  if (n > 0 && ordered) {
+    HIP_OK(hipEventCreateWithFlags(&r->after, hipEventDisableTiming));
+    const hipError_t e = hipEventRecord(r->after, (hipStream_t)stream);
+    if (e != hipSuccess) {
+      (void)hipEventDestroy(r->after);
+      return hip_fail(e, "hipEventRecord(request order)");
+    }
+  }
   std::lock_guard<std::mutex> g(S->mu);
-  if (S->stop) return GMMS_ERROR_INVALID_VALUE;
+  if (S->stop) {
+    if (r->after) (void)hipEventDestroy(r->after);
+    return GMMS_ERROR_INVALID_VALUE;
+  }
   r->id = S->next_id++;
   *ticket = r->id;
   S->st.submitted++;

@@ -180,6 +180,8 @@ def lib() -> ctypes.CDLL:
         "gmms_server_request_bytes": (i32, [i32, i64, ctypes.POINTER(sz)]),
         "gmms_server_create": (i32, [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]),
         "gmms_server_submit": (i32, [vp, i32, vp, vp, i64, vp, vp, ctypes.POINTER(ctypes.c_uint64)]),
+        "gmms_server_submit_after": (i32, [vp, i32, vp, vp, i64, vp, vp, vp, i32,
+                                           ctypes.POINTER(ctypes.c_uint64)]),
         "gmms_server_wait": (i32, [vp, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(i32), ctypes.POINTER(i32),
                                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]),
         "gmms_server_drain": (i32, [vp]),
@@ -315,16 +317,37 @@ class Workspace:
 
     tensor: object = None
 
-    def ensure(self, nbytes: int, device) -> Tuple[Optional[int], int]:
+    def ensure(self, nbytes: int, device, stream=None) -> Tuple[Optional[int], int]:
+        """A device pointer to at least nbytes; a new block is allocated on
+        `stream` (default: the current stream), the stream that uses it."""
         torch = _torch()
         if nbytes == 0:
             return None, 0
         if self.tensor is None or self.tensor.numel() < nbytes or self.tensor.device != device:
-            self.tensor = torch.empty(nbytes, dtype=torch.uint8, device=device)
+            self.tensor = None
+            with torch.cuda.stream(stream if stream is not None else torch.cuda.current_stream(device)):
+                self.tensor = torch.empty(nbytes, dtype=torch.uint8, device=device)
         return self.tensor.data_ptr(), self.tensor.numel()
 
 
-_default_ws = Workspace()
+# default workspaces, one per (device, stream): calls on different streams
+# never share one (calls on one stream are ordered by the stream). Calls from
+# several threads on the same stream must pass their own Workspace.
+_default_ws = {}
+
+
+def _workspace_for(workspace: Optional[Workspace], device, stream) -> Workspace:
+    if workspace is not None:
+        return workspace
+    key = (str(device), _stream_ptr(stream))
+    ws = _default_ws.get(key)
+    if ws is None:
+        ws = _default_ws[key] = Workspace()
+    return ws
+
+
+def _same_stream(a, b) -> bool:
+    return a.cuda_stream == b.cuda_stream and a.device == b.device
 
 
 def sort_inplace(keys, keys_alt, values=None, values_alt=None, *, dtype: Optional[int] = None,
@@ -335,9 +358,9 @@ def sort_inplace(keys, keys_alt, values=None, values_alt=None, *, dtype: Optiona
     _check_dev(keys, keys_alt, values, values_alt)
     dt = dtype if dtype is not None else (PAIR_U64_U32 if values is not None else dtype_of(keys))
     n = keys.numel()
-    ws = workspace or _default_ws
+    ws = _workspace_for(workspace, keys.device, stream)
     nbytes = workspace_size(dt, n, params)
-    wp, wb = ws.ensure(nbytes, keys.device)
+    wp, wb = ws.ensure(nbytes, keys.device, stream)
     alt = ctypes.c_int(0)
     st = lib().gmms_sort(dt, _ptr(keys), _ptr(keys_alt), _ptr(values), _ptr(values_alt), n, _pp(params),
                          wp, wb, _stream_ptr(stream), ctypes.byref(alt))
@@ -351,17 +374,23 @@ def sort(keys, values=None, *, params: Optional[Params] = None, workspace: Optio
     and values uint32/int32 viewed as uint32, ordered by (key, value).
     Returns the sorted keys, or (keys, values)."""
     torch = _torch()
-    k = keys.contiguous().clone()
-    ka = torch.empty_like(k)
-    if values is None:
-        alt = sort_inplace(k, ka, dtype=dtype_of(k), params=params, workspace=workspace, stream=stream)
-        return ka if alt else k
-    v = values.contiguous().clone()
-    va = torch.empty_like(v)
-    if k.element_size() != 8 or v.element_size() != 4 or k.numel() != v.numel():
-        raise GmmsError(ERROR_INVALID_VALUE, "pairs need 64-bit keys and 32-bit values of equal length")
-    alt = sort_inplace(k, ka, v, va, dtype=PAIR_U64_U32, params=params, workspace=workspace, stream=stream)
-    return (ka, va) if alt else (k, v)
+    cur = torch.cuda.current_stream(keys.device)
+    s = stream if stream is not None else cur
+    if not _same_stream(s, cur):
+        s.wait_stream(cur)  # the inputs come from work queued on the current stream
+    # copies, buffers and the sort all on s: the results belong to s
+    with torch.cuda.stream(s):
+        k = keys.contiguous().clone()
+        ka = torch.empty_like(k)
+        if values is None:
+            alt = sort_inplace(k, ka, dtype=dtype_of(k), params=params, workspace=workspace, stream=s)
+            return ka if alt else k
+        v = values.contiguous().clone()
+        va = torch.empty_like(v)
+        if k.element_size() != 8 or v.element_size() != 4 or k.numel() != v.numel():
+            raise GmmsError(ERROR_INVALID_VALUE, "pairs need 64-bit keys and 32-bit values of equal length")
+        alt = sort_inplace(k, ka, v, va, dtype=PAIR_U64_U32, params=params, workspace=workspace, stream=s)
+        return (ka, va) if alt else (k, v)
 
 
 def multimergesort(input, output, host_array=None, thread_blocks: int = 0, n: Optional[int] = None,
@@ -394,12 +423,12 @@ def merge_round(in_keys, out_keys, list_len: int, in_values=None, out_values=Non
     _check_dev(in_keys, out_keys, in_values, out_values)
     dt = dtype if dtype is not None else (PAIR_U64_U32 if in_values is not None else dtype_of(in_keys))
     n = in_keys.numel()
-    ws = workspace or _default_ws
+    ws = _workspace_for(workspace, in_keys.device, stream)
     b = ctypes.c_size_t()
     _check(lib().gmms_round_workspace_size(dt, n, list_len, _pp(params), ctypes.byref(b)),
            "gmms_round_workspace_size")
     nbytes = b.value
-    wp, wb = ws.ensure(nbytes, in_keys.device)
+    wp, wb = ws.ensure(nbytes, in_keys.device, stream)
     st = lib().gmms_merge_round(dt, _ptr(in_keys), _ptr(in_values), _ptr(out_keys), _ptr(out_values), n,
                                 list_len, _pp(params), int(to_external), wp, wb, _stream_ptr(stream))
     _check(st, "gmms_merge_round")
@@ -500,8 +529,8 @@ class SegPlan:
         1 if in keys_alt/values_alt. Asynchronous; keep the plan alive until
         the stream has run it."""
         _check_dev(keys, keys_alt, values, values_alt)
-        ws = workspace or _default_ws
-        wp, wb = ws.ensure(self.info.workspace_bytes, keys.device)
+        ws = _workspace_for(workspace, keys.device, stream)
+        wp, wb = ws.ensure(self.info.workspace_bytes, keys.device, stream)
         alt = ctypes.c_int(0)
         st = lib().gmms_seg_sort(self._h, _ptr(keys), _ptr(keys_alt), _ptr(values), _ptr(values_alt), wp, wb,
                                  _stream_ptr(stream), ctypes.byref(alt))
@@ -667,6 +696,10 @@ class Server:
         _check(lib().gmms_server_create(ctypes.byref(cfg), ctypes.byref(h)), "gmms_server_create")
         self._h = h
         self.config = cfg
+        # every request not yet reported done, by id: its Ticket holds the
+        # buffers the server reads and writes, so they stay alive until the
+        # request has finished even when the caller drops the ticket
+        self._live = {}
 
     def submit(self, keys, values=None, *, dtype: Optional[int] = None, out_keys=None,
                out_values=None) -> Ticket:
@@ -698,11 +731,18 @@ class Server:
         if values is not None:
             values = values.contiguous()
             out_v = out_like(values, out_values)
+        # device buffers: the server's copies wait for the work queued so far
+        # on torch's current stream (which may still be writing the inputs,
+        # including the copy contiguous() made, or using the result blocks)
+        on_dev = keys.is_cuda or out_k.is_cuda
+        stream = torch.cuda.current_stream().cuda_stream if on_dev else None
         tid = ctypes.c_uint64(0)
-        st = lib().gmms_server_submit(self._h, dt, keys.data_ptr(), _ptr(values), keys.numel(), out_k.data_ptr(),
-                                      _ptr(out_v), ctypes.byref(tid))
-        _check(st, "gmms_server_submit")
-        return Ticket(tid.value, keys, values, out_k, out_v)
+        st = lib().gmms_server_submit_after(self._h, dt, keys.data_ptr(), _ptr(values), keys.numel(),
+                                            out_k.data_ptr(), _ptr(out_v), stream, int(on_dev), ctypes.byref(tid))
+        _check(st, "gmms_server_submit_after")
+        t = Ticket(tid.value, keys, values, out_k, out_v)
+        self._live[t.id] = t
+        return t
 
     def wait(self, t: Ticket, timeout: Optional[float] = None) -> bool:
         """Wait for a request; True once it is done (then t.status,
@@ -717,6 +757,7 @@ class Server:
         if not done.value:
             return False
         t.status, t.latency_ms, t.batch = status.value, lat.value, batch.value
+        self._live.pop(t.id, None)
         return True
 
     def result(self, t: Ticket):
@@ -726,7 +767,11 @@ class Server:
         return t.out_keys if t.out_values is None else (t.out_keys, t.out_values)
 
     def drain(self) -> None:
+        """Wait for every request; afterwards each ticket has its status and
+        the server holds no request buffers."""
         _check(lib().gmms_server_drain(self._h), "gmms_server_drain")
+        for t in list(self._live.values()):
+            self.wait(t, timeout=0.0)
 
     def stats(self) -> ServerStats:
         s = ServerStats()
@@ -735,8 +780,9 @@ class Server:
 
     def close(self) -> None:
         if getattr(self, "_h", None):
-            lib().gmms_server_destroy(self._h)
+            lib().gmms_server_destroy(self._h)  # finishes every request first
             self._h = None
+            self._live.clear()
 
     def __enter__(self):
         return self

@@ -338,6 +338,15 @@ int gmms_server_create(const gmms_server_config* cfg, gmms_server** server);
  * identifies it. Returns at once. */
 int gmms_server_submit(gmms_server* server, int dtype, const void* keys, const void* vals, int64_t n,
                        void* out_keys, void* out_vals, uint64_t* ticket);
+/* The same, ordered after the caller's stream when `ordered` is 1: the
+ * server records an event on `stream` (a hipStream_t; NULL is the null
+ * stream) at submission, and the request's copies into and out of the arena
+ * wait for it. Use it for device buffers that work still queued on that
+ * stream writes (the inputs) or still reads (the result buffers).
+ * gmms_server_submit is this call with ordered = 0: the caller has finished
+ * every such work before it submits. */
+int gmms_server_submit_after(gmms_server* server, int dtype, const void* keys, const void* vals, int64_t n,
+                             void* out_keys, void* out_vals, void* stream, int ordered, uint64_t* ticket);
 /* Wait up to timeout_s seconds (< 0: forever) for a request; *done = 1 once
  * it has finished with status *status; latency_ms (may be NULL) receives
  * submission-to-completion time and batch_id (may be NULL) the batch that
@@ -345,7 +354,9 @@ int gmms_server_submit(gmms_server* server, int dtype, const void* keys, const v
  * reports it done. */
 int gmms_server_wait(gmms_server* server, uint64_t ticket, double timeout_s, int* done, int* status,
                      double* latency_ms, int64_t* batch_id);
-/* Wait until every submitted request has finished. */
+/* Wait until every submitted request has finished. Finished requests stay
+ * known to the server until a gmms_server_wait reports them done, so wait
+ * on each ticket once (timeout 0 after a drain) to release it. */
 int gmms_server_drain(gmms_server* server);
 int gmms_server_stats_get(gmms_server* server, gmms_server_stats* stats);
 void gmms_server_destroy(gmms_server* server);

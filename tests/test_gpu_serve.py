@@ -47,6 +47,13 @@ def _host(t):
     return t.numpy()
 
 
+def _prepared(gmms, kind, k, v):
+    """(keys, values, dtype) as pinned host tensors, ready to submit."""
+    if kind == "pair":
+        return _pinned(k.view(np.int64)), _pinned(v.view(np.int32)), gmms.PAIR_U64_U32
+    return _pinned(k), None, None
+
+
 def test_results_same_alone_and_in_mix(gmms, oracle, dev):
     print("server: alone vs mix", flush=True)
     reqs = _requests(oracle)
@@ -55,13 +62,20 @@ def test_results_same_alone_and_in_mix(gmms, oracle, dev):
         for kind, k, v in reqs:
             t = _submit(srv, kind, k, v)
             alone.append(_host(srv.result(t)))
-        # all at once: same-dtype requests share batches, batches overlap
-        ts = [_submit(srv, kind, k, v) for kind, k, v in reqs]
+        assert srv.stats().failed == 0
+    # all at once, queued behind a large request on a one-slot server: they
+    # arrive while it runs, so same-dtype requests share a batch when it ends
+    prep = [_prepared(gmms, kind, k, v) for kind, k, v in reqs]
+    blocker = _pinned(oracle.gen_i32(999, 1 << 26, 32))
+    with gmms.Server(hbm_budget=1 << 30, max_inflight=1) as srv:
+        tb = srv.submit(blocker)
+        ts = [srv.submit(pk, pv, dtype=dt) for pk, pv, dt in prep]
         mixed = [_host(srv.result(t)) for t in ts]
+        srv.result(tb)
         st = srv.stats()
-        assert st.completed == 2 * len(reqs) and st.failed == 0
+        assert st.completed == len(reqs) + 1 and st.failed == 0
         assert st.max_batch_size >= 2
-        assert len({t.batch for t in ts}) < len(ts)
+        assert len({t.batch for t in ts}) <= 3  # one batch per dtype
     for (kind, k, v), a, m in zip(reqs, alone, mixed):
         if kind == "pair":
             wk, wv = oracle.sort_pairs(k, v)
@@ -192,3 +206,50 @@ def test_wait_timeout_and_drain(gmms, oracle, dev):
         for t in ts:
             assert srv.wait(t, timeout=0.0) and t.status == 0 and t.latency_ms > 0
         assert srv.stats().completed == 8
+
+
+def test_device_inputs_ordered_after_current_stream(gmms, oracle, dev):
+    """A device request whose input is still being written by work queued on
+    torch's current stream (here the contiguous copy of a strided view,
+    queued behind a few milliseconds of other kernels) is read only after
+    that work: the server's copies wait for the stream."""
+    import torch
+
+    print("server: device input produced by queued work", flush=True)
+    n = 1 << 21
+    a = oracle.gen_i32(4000, n, 32)
+    base = torch.from_numpy(a).to(dev)
+    torch.cuda.synchronize()
+    with gmms.Server(hbm_budget=256 << 20) as srv:
+        work = torch.empty(1 << 27, device=dev)
+        for _ in range(8):
+            work.normal_()  # queued ahead of the producer below
+        x = base.flip(0)  # written by a kernel still queued behind the chain
+        t = srv.submit(x[::2])  # strided: submit copies it on the current stream
+        got = srv.result(t).cpu().numpy()
+    assert np.array_equal(got, oracle.sort(a[::-1][::2].copy()))
+
+
+def test_dropped_tickets_keep_buffers_alive(gmms, oracle, dev):
+    """Requests whose tickets the caller drops keep their buffers alive
+    until they finish: tensors allocated meanwhile are never written by the
+    server, and drain() reports every request done."""
+    import gc
+
+    import torch
+
+    print("server: dropped tickets", flush=True)
+    n = 1 << 20
+    srcs = [oracle.gen_i32(5000 + i, n, 32) for i in range(12)]
+    with gmms.Server(hbm_budget=256 << 20, max_inflight=2) as srv:
+        for a in srcs:
+            srv.submit(torch.from_numpy(a).to(dev)[::1])  # device input and result: ticket dropped
+        gc.collect()
+        fills = [torch.full((n,), 12345 + i, dtype=torch.int32, device=dev) for i in range(12)]
+        srv.drain()
+        torch.cuda.synchronize()
+        for i, f in enumerate(fills):
+            assert bool((f == 12345 + i).all())
+        st = srv.stats()
+        assert st.completed == len(srcs) and st.failed == 0
+        assert not srv._live
