@@ -150,3 +150,118 @@ def test_basecase_and_round_oracle(oracle):
         assert np.array_equal(b[t:t + 1024], np.sort(a[t:t + 1024]))
     r1 = oracle.merge_round_i32(b, 4, 1024)
     assert np.array_equal(r1[:4096], np.sort(a[:4096])) and np.array_equal(r1[4096:], np.sort(a[4096:]))
+
+
+@pytest.mark.parametrize("threads", [1, 4, 0])
+def test_psort_i32_matches_numpy(oracle, threads):
+    """oracle_psort_i32 (libstdc++ parallel sort): the bit-exact checker of
+    the 2^28 benchmark output and the CPU baseline. Pinned by numpy's sort
+    on the same bits, with 1, 4 and all host threads."""
+    rng = np.random.default_rng(21 + threads)
+    cases = [np.zeros(0, np.int32), np.array([7], np.int32),
+             rng.integers(-2**31, 2**31, 4097, dtype=np.int64).astype(np.int32),
+             oracle.gen_i32(0x6D6D7301, 1 << 20, 32),
+             rng.integers(0, 3, 1 << 20).astype(np.int32),  # duplicate-heavy
+             np.full(100_000, -2**31, np.int32)]
+    for a in cases:
+        b = a.copy()
+        oracle.lib.oracle_psort_i32(b.ctypes.data, b.size, threads)
+        assert np.array_equal(b, np.sort(a)), (a.size, threads)
+
+
+@pytest.mark.parametrize("threads", [1, 4, 0])
+def test_psort_u64pairs_matches_lexsort(oracle, threads):
+    """oracle_psort_u64pairs: (key, value) lexicographic, pinned by numpy's
+    lexsort, including keys with many equal copies and equal values."""
+    rng = np.random.default_rng(31 + threads)
+    for n in (0, 1, 4097, 1 << 20):
+        k = rng.integers(0, 2**64, n, dtype=np.uint64)
+        k[::3] = k[0] if n else k[::3]  # a third of the keys equal
+        v = rng.integers(0, 8, n, dtype=np.uint64).astype(np.uint32)
+        k2, v2 = k.copy(), v.copy()
+        oracle.lib.oracle_psort_u64pairs(k2.ctypes.data, v2.ctypes.data, n, threads)
+        order = np.lexsort((v, k))
+        assert np.array_equal(k2, k[order]) and np.array_equal(v2, v[order]), (n, threads)
+
+
+def test_time_sort_reports_positive_medians(oracle):
+    a = oracle.gen_i32(3, 1 << 16, 32)
+    for threads in (1, 2):
+        t = oracle.lib.oracle_time_sort_i32(a.ctypes.data, a.size, threads, 3)
+        assert 0.0 < t < 10.0
+
+
+def test_normal_generator_distribution(oracle):
+    """gen_f32_normal is normal(0, 1): Kolmogorov-Smirnov against the
+    standard normal CDF, sample mean and variance within 5 standard errors
+    on 10^6 draws."""
+    from scipy import stats
+
+    n = 10**6
+    x = oracle.gen_f32_normal(0x6D6D7304, n).astype(np.float64)
+    assert np.isfinite(x).all()
+    assert abs(x.mean()) < 5 / math.sqrt(n)
+    assert abs(x.var() - 1.0) < 5 * math.sqrt(2.0 / n)
+    ks = stats.kstest(x, "norm")
+    assert ks.pvalue > 1e-3, ks
+    # tails: P(|x| > 3) = 0.0027
+    assert abs((np.abs(x) > 3).mean() - 0.0026998) < 5 * math.sqrt(0.0027 / n)
+
+
+def test_zipf_generator_rank_frequency(oracle):
+    """Zipf(1.1) keys over 2^20 values: the log-log rank-frequency slope
+    over the 1000 most frequent keys is -1.1, and the most frequent key
+    takes the share 1 / H(2^20, 1.1) of the draws."""
+    n = 1 << 20
+    k, v = oracle.gen_pairs(0x6D6D7313, n, 3, 1.1, 1 << 20)
+    assert np.array_equal(v, np.arange(n, dtype=np.uint32))
+    _, counts = np.unique(k, return_counts=True)
+    counts = np.sort(counts)[::-1].astype(np.float64)
+    r = np.arange(1, 1001, dtype=np.float64)
+    slope = np.polyfit(np.log(r), np.log(counts[:1000]), 1)[0]
+    assert abs(slope + 1.1) < 0.05, slope
+    h = float(np.sum(np.arange(1, (1 << 20) + 1, dtype=np.float64) ** -1.1))
+    share = counts[0] / n
+    assert abs(share - 1.0 / h) < 5 * math.sqrt(share / n), (share, 1.0 / h)
+
+
+def test_sorted_and_reverse_pairs_are_the_uniform_draw(oracle):
+    """Distributions 1 and 2 are the uniform draw (distribution 0, same
+    seed) sorted ascending and descending; values are the indexes."""
+    n = 100_000
+    k0, v0 = oracle.gen_pairs(77, n, 0, 1.1, 1 << 20)
+    k1, v1 = oracle.gen_pairs(77, n, 1, 1.1, 1 << 20)
+    k2, v2 = oracle.gen_pairs(77, n, 2, 1.1, 1 << 20)
+    assert np.array_equal(k1, np.sort(k0)) and np.array_equal(k2, np.sort(k0)[::-1])
+    for v in (v0, v1, v2):
+        assert np.array_equal(v, np.arange(n, dtype=np.uint32))
+
+
+def test_seglens_log_uniform(oracle):
+    """gen_seglens: log2 of the lengths is uniform on [10, 22) (KS test),
+    the lengths are integers in [2^10, 2^22)."""
+    from scipy import stats
+
+    lens = oracle.gen_seglens(0x6D6D7303, 8192, 10.0, 22.0)
+    assert lens.min() >= 1 << 10 and lens.max() < 1 << 22
+    # floor() shifts log2 down by < 1/len: negligible against the KS bound
+    ks = stats.kstest((np.log2(lens) - 10.0) / 12.0, "uniform")
+    assert ks.pvalue > 1e-3, ks
+    # the literal 8192-segment batch: about 4.06e9 keys
+    assert 4.0e9 < lens.sum() < 4.12e9
+
+
+def test_uniform_int32_generator_spreads(oracle):
+    """(int32)(u64 >> 32) is uniform over the full int32 range: a chi-square
+    test over 256 buckets of the top byte on 2^20 draws; shift 34 stays in
+    [0, 2^30) and shift 54 in [0, 1024) with every value present."""
+    from scipy import stats
+
+    a = oracle.gen_i32(0x6D6D7301, 1 << 20, 32)
+    top = (a.view(np.uint32) >> 24).astype(np.int64)
+    chi = stats.chisquare(np.bincount(top, minlength=256))
+    assert chi.pvalue > 1e-3, chi
+    b = oracle.gen_i32(0x6D6D7302, 1 << 20, 34)
+    assert b.min() >= 0 and b.max() < 1 << 30 and b.max() > (1 << 30) - (1 << 14)
+    c = oracle.gen_i32(0x6D6D7302, 1 << 20, 54)
+    assert np.array_equal(np.unique(c), np.arange(1024, dtype=np.int32))
