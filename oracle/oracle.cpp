@@ -65,7 +65,7 @@ void oracle_gen_pairs(uint64_t seed, int64_t n, int dist, double zipf_s,
     for (int64_t i = 0; i < n; ++i) keys[i] = g();
   } else if (dist == 1 || dist == 2) {
     for (int64_t i = 0; i < n; ++i) keys[i] = g();
-    std::sort(keys, keys + n);
+    __gnu_parallel::sort(keys, keys + n);  // the same result as std::sort
     if (dist == 2) std::reverse(keys, keys + n);
   } else {
     std::vector<double> cdf((size_t)distinct);
@@ -75,11 +75,14 @@ void oracle_gen_pairs(uint64_t seed, int64_t n, int dist, double zipf_s,
       cdf[(size_t)r] = acc;
     }
     for (auto& c : cdf) c /= acc;
+    // the draws in stream order, then the inverse-CDF lookups in parallel
+    for (int64_t i = 0; i < n; ++i) keys[i] = g();
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) {
-      double u = u53(g());
+      const double u = u53(keys[i]);
       int64_t r = std::lower_bound(cdf.begin(), cdf.end(), u) - cdf.begin();
       if (r >= distinct) r = distinct - 1;
-      uint64_t rank = (uint64_t)r + 1;
+      const uint64_t rank = (uint64_t)r + 1;
       keys[i] = (rank * 0x9E3779B97F4A7C15ull) ^ 0xD1B54A32D192ED03ull;
     }
   }

@@ -157,3 +157,85 @@ def test_uint64_2p27(gmms, dev):
     assert _same_multiset_u32(lo(k), lo(y)) and _same_multiset_u32(hi(k), hi(y))
     assert _same_multiset_u32(mix(k), mix(y))
     assert torch.equal(sort_u64(y), y)
+
+
+@pytest.mark.parametrize("dist", [0, 1, 2, 3], ids=["uniform", "sorted", "reverse", "zipf1.1"])
+def test_pairs_2p27_baseline_distributions(gmms, oracle, dev, dist):
+    """The benchmark's pair inputs at full size, from the oracle's own
+    generators (seed 0x6D6D7310 + dist): uniform 64-bit keys, the same keys
+    sorted and reverse sorted, and Zipf(1.1) over 2^20 values; value = index.
+    The value-is-index proof above fixes the output exactly."""
+    import numpy as np
+    import torch
+
+    n = 1 << 27
+    print(f"full pairs n={n} oracle distribution {dist}", flush=True)
+    hk, hv = oracle.gen_pairs(0x6D6D7310 + dist, n, dist, 1.1, 1 << 20)
+    k = torch.from_numpy(hk.view(np.int64)).to(dev)
+    v = torch.from_numpy(hv.view(np.int32)).to(dev)
+    del hk, hv
+    ks, vs = gmms.sort(k, v)
+    idx = _u32(vs)
+    assert torch.equal(torch.bincount(idx, minlength=n), torch.ones(n, dtype=torch.int64, device=dev))
+    assert torch.equal(ks, k[idx])
+    s = ks ^ SIGN64
+    strictly = (s[:-1] < s[1:]) | ((s[:-1] == s[1:]) & (idx[:-1] < idx[1:]))
+    assert bool(strictly.all())
+
+
+def test_batched_literal_8192_segments(gmms, oracle, dev):
+    """The batched configuration as written: 8192 int32 segments with
+    lengths log-uniform in [2^10, 2^22] (oracle seed 0x6D6D7303, about
+    4.06e9 keys, more than 2^31), values uniform in [0, 2^31) generated on
+    the device, sorted as one batch. Checked per segment at full size:
+    sortedness inside every segment and the same multiset (sums of the words,
+    two non-linear mixes and products of their halves, per segment); and 64
+    segments (the 8 longest and 56 drawn at random) bit for bit against the
+    oracle and against the plain sort of that segment alone."""
+    import numpy as np
+    import torch
+
+    lens = oracle.gen_seglens(0x6D6D7303, 8192, 10, 22)
+    offs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    total = int(offs[-1])
+    assert total > (1 << 31)
+    print(f"batched literal: 8192 segments, {total} keys", flush=True)
+    x = torch.randint(0, 1 << 31, (total,), dtype=torch.int32, device=dev, generator=_gen(0x6D6D7303, dev))
+    y = gmms.segmented_sort(x, offs)
+    torch.cuda.synchronize()
+    print("batched literal: sorted; checking", flush=True)
+    # chunks of whole segments, up to 2^28 keys each
+    s0 = 0
+    while s0 < len(lens):
+        s1 = s0
+        while s1 < len(lens) and offs[s1 + 1] - offs[s0] <= (1 << 28):
+            s1 += 1
+        a, b = int(offs[s0]), int(offs[s1])
+        xs, ys = x[a:b], y[a:b]
+        seg_lens = torch.from_numpy(lens[s0:s1]).to(dev)
+        seg = torch.repeat_interleave(torch.arange(s1 - s0, device=dev), seg_lens)
+        # sorted inside each segment (a boundary may step down)
+        up = ys[1:] >= ys[:-1]
+        first = torch.from_numpy(offs[s0 + 1:s1] - a).to(dev)  # first index of segments s0+1 .. s1-1
+        up[first - 1] = True
+        assert bool(up.all()), f"segments {s0}..{s1}: not sorted"
+        for tx, ty in zip(_seg_terms(_u32(xs)), _seg_terms(_u32(ys))):
+            sx = torch.zeros(s1 - s0, dtype=torch.int64, device=dev).index_add_(0, seg, tx)
+            sy = torch.zeros(s1 - s0, dtype=torch.int64, device=dev).index_add_(0, seg, ty)
+            assert torch.equal(sx, sy), f"segments {s0}..{s1}: multiset differs"
+        del xs, ys, seg, up
+        s0 = s1
+    rng = np.random.default_rng(0x6D6D7303)
+    pick = list(np.argsort(lens)[-8:]) + list(rng.choice(len(lens), 56, replace=False))
+    for s in pick:
+        a, b = int(offs[s]), int(offs[s + 1])
+        src = x[a:b].cpu().numpy()
+        got = y[a:b].cpu().numpy()
+        assert np.array_equal(got, oracle.sort(src)), f"segment {s} differs from the oracle"
+        assert torch.equal(gmms.sort(x[a:b]), y[a:b]), f"segment {s} differs from its plain sort"
+
+
+def _seg_terms(words):
+    """The order-independent terms of _hash_sums, per element (each < 2^32)."""
+    lo, hi = words & 0xFFFF, words >> 16
+    return (words, _fmix(words, 0x7FEB352D, 0x446CA68B), _fmix(words, 0x2C1B3C6D, 0x297A2D39), lo * lo, lo * hi)

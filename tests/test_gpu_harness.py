@@ -77,3 +77,28 @@ def test_race_repeat(gmms, oracle, dev):
         k.copy_(src)
         alt = gmms.sort_inplace(k, ka, workspace=ws)
         assert torch.equal(ka if alt else k, want), f"run {it}"
+
+
+def test_all_sizes_sweep_to_2p26(gmms, dev):
+    """scripts/allSizes.sh:25-32 in full: 100 sizes per octave,
+    n = 2^k + r * 2^k / 32768 with r uniform in [0, 32768), k = 10..26
+    (1700 sorts). Keys are drawn on the device, alternately over the full
+    int32 range and from [0, 1024) (the reference harness's rand() % RANGE);
+    each result is compared bit for bit with torch.sort of the same keys, an
+    independent library sort (unique output for integer keys)."""
+    import torch
+
+    rng = np.random.default_rng(0xA115)
+    g = torch.Generator(device=dev)
+    g.manual_seed(0xA115)
+    ws = gmms.Workspace()
+    for k in range(10, 27):
+        print(f"all sizes: octave 2^{k}", flush=True)
+        for i in range(100):
+            n = (1 << k) + int(rng.integers(0, 32768)) * (1 << k) // 32768
+            if i % 2:
+                x = torch.randint(0, 1024, (n,), dtype=torch.int32, device=dev, generator=g)
+            else:
+                x = torch.randint(-(1 << 31), (1 << 31) - 1, (n,), dtype=torch.int32, device=dev, generator=g)
+            got = gmms.sort(x, workspace=ws)
+            assert torch.equal(got, torch.sort(x).values), (k, i, n)
