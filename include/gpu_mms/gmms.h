@@ -96,7 +96,7 @@ int gmms_element_size(int dtype, size_t* key_bytes, size_t* value_bytes);
 /* Number of merge rounds for n elements: iterations of the reference's
  * `for (listSize = M; listSize < N; listSize *= K)` (multimergesort.hxx:87),
  * M = 1024. */
-int gmms_num_rounds(int64_t n, int K, int* rounds);
+int gmms_num_rounds(int64_t n, int kway, int* rounds);
 /* Merge rounds gmms_sort runs for n elements with these knobs: the same loop
  * from listSize = the resolved base_run (so one round fewer than
  * gmms_num_rounds whenever the 4096-element base runs save one). */

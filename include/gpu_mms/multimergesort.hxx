@@ -52,6 +52,24 @@ inline int last_status_storage(int set = -1) {
   return s;
 }
 
+// The knobs of params.h, checked as the reference's build assumes them
+// (PL = log2 K, src/params.h:19-20): K a power of two in 2..64.
+static_assert(K >= 2 && K <= 64 && (K & (K - 1)) == 0, "params.h: K must be a power of two in 2..64");
+static_assert((1 << (PL)) == K, "params.h: PL must be log2(K)");
+static_assert(MERGE_KERNEL >= 0 && MERGE_KERNEL <= 2, "params.h: MERGE_KERNEL is 0, 1 or 2");
+
+// The gmms_params the drop-in template runs with: params.h's K, merge
+// kernel (PIPELINE / MERGE_KERNEL) and BLOCKS, or threadBlocks when > 0.
+inline gmms_params drop_in_params(int threadBlocks, int descending = 0) {
+  gmms_params p;
+  gmms_get_default_params(&p);
+  p.kway = K;
+  p.pipeline = MERGE_KERNEL;
+  p.mergers = threadBlocks > 0 ? threadBlocks : BLOCKS;
+  p.descending = descending;
+  return p;
+}
+
 }  // namespace gpu_mms
 
 /* Status of the calling thread's last multimergesort call. */
@@ -63,12 +81,7 @@ T* multimergesort(T* input, T* output, T* h_data, int threadBlocks, int64_t N) {
   static_assert(gpu_mms::order_of<f>::value >= 0,
                 "libgmms.so runs cmp<DATATYPE> (a < b) or greater<DATATYPE> (a > b)");
   void* result = nullptr;
-  gmms_params p;
-  gmms_get_default_params(&p);
-  p.kway = K;
-  p.pipeline = MERGE_KERNEL;
-  p.mergers = threadBlocks > 0 ? threadBlocks : BLOCKS;
-  p.descending = gpu_mms::order_of<f>::value;
+  const gmms_params p = gpu_mms::drop_in_params(threadBlocks, gpu_mms::order_of<f>::value);
   const int st = gmms_multimergesort_params(gpu_mms::dtype_of<T>::value, input, output, h_data, &p, N, &result);
   gpu_mms::last_status_storage(st);
   return st == GMMS_SUCCESS ? static_cast<T*>(result) : nullptr;

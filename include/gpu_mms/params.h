@@ -1,15 +1,25 @@
 /*
  * params.h -- compile-time defaults of the drop-in template, same knob names
- * as the reference's src/params.h:18-38. They only seed gmms_params; every
- * value can also be chosen at run time through the C ABI (gmms.h).
+ * as the reference's src/params.h:18-38. They seed the gmms_params that
+ * multimergesort<T,cmp> passes to the library; every value can also be
+ * chosen at run time through the C ABI (gmms.h).
  *
- *   K         lists merged per round (reference: 64)
- *   PL        log2(K): heap levels = wavefronts per merger of the heap kernel
- *   PIPELINE  defined as in the reference (the minBlockHeap kernel is always
- *             the pipelined one: one wavefront per heap level)
- *   MERGE_KERNEL  which K-way merge runs the rounds: 0 tile merge with one
- *             merger per wavefront (default, fastest), 1 the pipelined
- *             minBlockHeap, 2 tile merge with one merger per workgroup
+ *   K         lists merged per round (reference: 64; here 16, the fastest
+ *             on MI355X for every measured size and dtype)
+ *   PL        log2(K): heap levels, i.e. wavefronts per merger of the
+ *             pipelined minBlockHeap kernel; derived from K unless given,
+ *             and checked against it (multimergesort.hxx)
+ *   PIPELINE  as in the reference: defined -> the pipelined minBlockHeap
+ *             merge, one wavefront per heap level (PL wavefronts per
+ *             merger, src/io-merge-gen.hxx:305-355); undefined -> one
+ *             wavefront per merger (src/io-merge-gen.hxx:357-381), here the
+ *             register-tile merger. Unlike the reference it is NOT defined
+ *             by default: on CDNA4 the one-wavefront merger is the fast one
+ *             (2^28 int32: 3.9 ms against 14.7 ms for the heap). -DPIPELINE
+ *             selects the heap; -DNO_PIPELINE is accepted and keeps it off.
+ *   MERGE_KERNEL  the merge kernel by number, overriding PIPELINE when
+ *             given: 0 one-wavefront tile merger, 1 pipelined minBlockHeap,
+ *             2 tile merge with one merger per workgroup
  *   RANGE     range of the harness's random keys, rand() % RANGE
  *   BLOCKS    mergers that run at once when a round has few tasks
  *             (reference: 482); a round splits its tasks into a whole
@@ -23,17 +33,22 @@
 #ifndef GPU_MMS_PARAMS_H
 #define GPU_MMS_PARAMS_H
 
+#if defined(PIPELINE) && defined(NO_PIPELINE)
+#error "params.h: define at most one of PIPELINE and NO_PIPELINE"
+#endif
+
 #ifndef MERGE_KERNEL
+#ifdef PIPELINE
+#define MERGE_KERNEL 1
+#else
 #define MERGE_KERNEL 0
+#endif
 #endif
 #ifndef K
 #define K 16
 #endif
 #ifndef PL
-#define PL 4
-#endif
-#ifndef NO_PIPELINE
-#define PIPELINE
+#define PL ((K) >= 64 ? 6 : (K) >= 32 ? 5 : (K) >= 16 ? 4 : (K) >= 8 ? 3 : (K) >= 4 ? 2 : 1)
 #endif
 #ifndef RANGE
 #define RANGE 1024
