@@ -243,6 +243,31 @@ __global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ src, u
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
 }
 
+// Copy-probe variants: U 16-byte loads per lane in flight before the U
+// stores (lane-consecutive within each of the U slices of a block's
+// stretch), optionally non-temporal stores; grid-stride over stretches.
+template <int U, bool NT>
+__global__ __launch_bounds__(256) void k_copy_u(const u32x4* __restrict__ src, u32x4* __restrict__ dst, int64_t n16) {
+  const int64_t span = (int64_t)U * blockDim.x;
+  const int64_t stride = (int64_t)gridDim.x * span;
+  for (int64_t b = (int64_t)blockIdx.x * span; b < n16; b += stride) {
+    u32x4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = b + (int64_t)u * blockDim.x + threadIdx.x;
+      if (i < n16) v[u] = src[i];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = b + (int64_t)u * blockDim.x + threadIdx.x;
+      if (i < n16) {
+        if constexpr (NT) __builtin_nontemporal_store(v[u], dst + i);
+        else dst[i] = v[u];
+      }
+    }
+  }
+}
+
 template <int DT>
 __global__ void k_pad(void* keys, uint32_t* vals, int64_t count) {
   using E = typename DType<DT>::E;
@@ -291,6 +316,34 @@ int gmms_copy(void* dst, const void* src, size_t bytes, void* stream) {
   const int64_t blocks = std::min<int64_t>((n16 + 255) / 256, (int64_t)cu_count() * 32);
   hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, (const uint4*)src,
                      (uint4*)dst, n16);
+  HIP_OK(hipGetLastError());
+  return GMMS_SUCCESS;
+}
+
+int gmms_copy_probe(void* dst, const void* src, size_t bytes, int unroll, int nontemporal, int blocks_per_cu,
+                    void* stream) {
+  if (!dst || !src || (bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15)) return GMMS_ERROR_INVALID_VALUE;
+  if (unroll != 1 && unroll != 2 && unroll != 4 && unroll != 8) return GMMS_ERROR_INVALID_VALUE;
+  if (blocks_per_cu < 0 || blocks_per_cu > 1024 || (nontemporal != 0 && nontemporal != 1))
+    return GMMS_ERROR_INVALID_VALUE;
+  if (!bytes) return GMMS_SUCCESS;
+  const int64_t n16 = (int64_t)(bytes / 16);
+  const int64_t need = (n16 + 256LL * unroll - 1) / (256LL * unroll);
+  const int64_t blocks = blocks_per_cu ? std::min<int64_t>(need, (int64_t)cu_count() * blocks_per_cu) : need;
+  const dim3 g((unsigned)blocks), b(256);
+  hipStream_t s = (hipStream_t)stream;
+  const u32x4* in = (const u32x4*)src;
+  u32x4* out = (u32x4*)dst;
+  switch (unroll * 2 + nontemporal) {
+    case 2: hipLaunchKernelGGL((k_copy_u<1, false>), g, b, 0, s, in, out, n16); break;
+    case 3: hipLaunchKernelGGL((k_copy_u<1, true>), g, b, 0, s, in, out, n16); break;
+    case 4: hipLaunchKernelGGL((k_copy_u<2, false>), g, b, 0, s, in, out, n16); break;
+    case 5: hipLaunchKernelGGL((k_copy_u<2, true>), g, b, 0, s, in, out, n16); break;
+    case 8: hipLaunchKernelGGL((k_copy_u<4, false>), g, b, 0, s, in, out, n16); break;
+    case 9: hipLaunchKernelGGL((k_copy_u<4, true>), g, b, 0, s, in, out, n16); break;
+    case 16: hipLaunchKernelGGL((k_copy_u<8, false>), g, b, 0, s, in, out, n16); break;
+    default: hipLaunchKernelGGL((k_copy_u<8, true>), g, b, 0, s, in, out, n16); break;
+  }
   HIP_OK(hipGetLastError());
   return GMMS_SUCCESS;
 }

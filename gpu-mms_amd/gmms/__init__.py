@@ -163,6 +163,7 @@ def lib() -> ctypes.CDLL:
         "gmms_pad": (i32, [i32, vp, vp, i64, vp]),
         "gmms_stage_timing": (i32, [i32]),
         "gmms_copy": (i32, [vp, vp, sz, vp]),
+        "gmms_copy_probe": (i32, [vp, vp, sz, i32, i32, i32, vp]),
         "gmms_stage_times": (i32, [vp, vp, vp, i32, ctypes.POINTER(i32)]),
         "gmms_seg_plan_create": (i32, [i32, vp, i64, P, ctypes.POINTER(vp)]),
         "gmms_seg_plan_info": (i32, [vp, ctypes.POINTER(SegInfo)]),
@@ -463,6 +464,14 @@ def copy(dst, src, stream=None) -> None:
     _check_dev(dst, src)
     nbytes = src.numel() * src.element_size()
     _check(lib().gmms_copy(_ptr(dst), _ptr(src), nbytes, _stream_ptr(stream)), "gmms_copy")
+
+
+def copy_probe(dst, src, unroll: int = 1, nontemporal: bool = False, blocks_per_cu: int = 0, stream=None) -> None:
+    """A variant of the copy probe (gmms_copy_probe)."""
+    _check_dev(dst, src)
+    nbytes = src.numel() * src.element_size()
+    _check(lib().gmms_copy_probe(_ptr(dst), _ptr(src), nbytes, unroll, int(nontemporal), blocks_per_cu,
+                                 _stream_ptr(stream)), "gmms_copy_probe")
 
 
 # ------------------------------------------------------------ stage timing

@@ -159,6 +159,13 @@ int gmms_multimergesort_params(int dtype, void* input, void* output, void* host_
  * 16-byte aligned). No counterpart in the reference: it is the stream-copy
  * probe that the HBM roofline of the sort is measured against. */
 int gmms_copy(void* dst, const void* src, size_t bytes, void* stream);
+/* Variants of that probe, to validate its figure: `unroll` (1, 2, 4, 8)
+ * 16-byte loads per lane in flight before their stores, non-temporal stores
+ * when `nontemporal` is 1, and at most blocks_per_cu workgroups of 256 lanes
+ * per compute unit (grid-stride beyond; 0: one workgroup per stretch of
+ * the array, no stride). No counterpart in the reference. */
+int gmms_copy_probe(void* dst, const void* src, size_t bytes, int unroll, int nontemporal, int blocks_per_cu,
+                    void* stream);
 
 /* ---- per-launch timing (the reference times the whole call with CUDA
  * events, src/testMM.cu:171-186) ---------------------------------------- */

@@ -169,6 +169,19 @@ void oracle_psort_i32(int32_t* a, int64_t n, int threads) {
   __gnu_parallel::sort(a, a + n);
 }
 
+void oracle_psort_u64(uint64_t* a, int64_t n, int threads) {
+  if (threads > 0) omp_set_num_threads(threads);
+  __gnu_parallel::sort(a, a + n);
+}
+
+void oracle_psort_f32(float* a, int64_t n, int threads) {
+  if (threads > 0) omp_set_num_threads(threads);
+  uint32_t* u = reinterpret_cast<uint32_t*>(a);
+  __gnu_parallel::sort(u, u + n, [](uint32_t x, uint32_t y) {
+    return oracle_f32_order_key(x) < oracle_f32_order_key(y);
+  });
+}
+
 void oracle_psort_u64pairs(uint64_t* keys, uint32_t* vals, int64_t n, int threads) {
   if (threads > 0) omp_set_num_threads(threads);
   std::vector<Pair> p((size_t)n);
@@ -226,6 +239,19 @@ void oracle_corank_i32(const int32_t* base, int64_t stride, const int64_t* lens,
 }
 
 // ------------------------------------------------------------- CPU baseline
+int oracle_time_sort_i32_runs(const int32_t* a, int64_t n, int threads, int warmups, int reps, double* secs) {
+  std::vector<int32_t> buf((size_t)n);
+  for (int it = 0; it < warmups + reps; ++it) {
+    std::memcpy(buf.data(), a, (size_t)n * sizeof(int32_t));  // a fresh copy per run, not timed
+    auto t0 = std::chrono::steady_clock::now();
+    if (threads == 1) std::sort(buf.begin(), buf.end());
+    else oracle_psort_i32(buf.data(), n, threads);
+    auto t1 = std::chrono::steady_clock::now();
+    if (it >= warmups) secs[it - warmups] = std::chrono::duration<double>(t1 - t0).count();
+  }
+  return std::is_sorted(buf.begin(), buf.end()) ? 0 : 1;
+}
+
 double oracle_time_sort_i32(const int32_t* a, int64_t n, int threads, int reps) {
   std::vector<int32_t> buf((size_t)n);
   std::vector<double> t;

@@ -71,6 +71,8 @@ void oracle_sort_pairs(uint64_t* keys, uint32_t* vals, int64_t n);
 /* parallel std::sort baseline (libstdc++ parallel mode), threads <= 0: all */
 void oracle_psort_i32(int32_t* a, int64_t n, int threads);
 void oracle_psort_u64pairs(uint64_t* keys, uint32_t* vals, int64_t n, int threads);
+void oracle_psort_u64(uint64_t* a, int64_t n, int threads);
+void oracle_psort_f32(float* a, int64_t n, int threads);        /* total order on bits */
 
 /* ---- base case (src/basecase/squareSort.hxx:62-131) -----------------------
  * Sort every contiguous tile of `tile` elements in place; the ragged tail
@@ -107,6 +109,10 @@ void oracle_corank_i32(const int32_t* base, int64_t stride, const int64_t* lens,
  * Returns median seconds of `reps` runs (after one warm-up) of std::sort
  * (threads == 1) or parallel sort on a fresh copy of `a`. */
 double oracle_time_sort_i32(const int32_t* a, int64_t n, int threads, int reps);
+/* The same with every run's seconds: `warmups` untimed runs, then `reps`
+ * timed ones into secs[0..reps); returns 0 when the last run is sorted. */
+int    oracle_time_sort_i32_runs(const int32_t* a, int64_t n, int threads, int warmups, int reps,
+                                 double* secs);
 int    oracle_hw_threads(void);
 
 #ifdef __cplusplus

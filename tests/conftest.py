@@ -44,6 +44,10 @@ class Oracle:
         L.oracle_sort_pairs.argtypes = [vp, vp, i64]
         L.oracle_psort_i32.argtypes = [vp, i64, i32]
         L.oracle_psort_u64pairs.argtypes = [vp, vp, i64, i32]
+        L.oracle_psort_u64.argtypes = [vp, i64, i32]
+        L.oracle_psort_f32.argtypes = [vp, i64, i32]
+        L.oracle_time_sort_i32_runs.restype = i32
+        L.oracle_time_sort_i32_runs.argtypes = [vp, i64, i32, i32, i32, vp]
         L.oracle_basecase_i32.argtypes = [vp, i64, i64]
         L.oracle_rounds.restype = i32
         L.oracle_rounds.argtypes = [i64, i64, i64]
@@ -101,6 +105,19 @@ class Oracle:
                 np.dtype(np.int64): "i64", np.dtype(np.uint64): "u64", np.dtype(np.float64): "f64"}[b.dtype]
         getattr(self.lib, f"oracle_sort_{name}")(self.p(b), b.size)
         return b
+
+    def psort(self, a, threads=0):
+        """Parallel CPU sort (all threads by default) of int32, uint64 or
+        float32 (total order) keys; a sorted copy."""
+        b = np.ascontiguousarray(a).copy()
+        name = {np.dtype(np.int32): "i32", np.dtype(np.uint64): "u64", np.dtype(np.float32): "f32"}[b.dtype]
+        getattr(self.lib, f"oracle_psort_{name}")(self.p(b), b.size, threads)
+        return b
+
+    def psort_pairs(self, k, v, threads=0):
+        k2, v2 = np.ascontiguousarray(k).copy(), np.ascontiguousarray(v).copy()
+        self.lib.oracle_psort_u64pairs(self.p(k2), self.p(v2), k2.size, threads)
+        return k2, v2
 
     def sort_pairs(self, k, v):
         k2, v2 = np.ascontiguousarray(k).copy(), np.ascontiguousarray(v).copy()

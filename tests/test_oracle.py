@@ -265,3 +265,28 @@ def test_uniform_int32_generator_spreads(oracle):
     assert b.min() >= 0 and b.max() < 1 << 30 and b.max() > (1 << 30) - (1 << 14)
     c = oracle.gen_i32(0x6D6D7302, 1 << 20, 54)
     assert np.array_equal(np.unique(c), np.arange(1024, dtype=np.int32))
+
+
+@pytest.mark.parametrize("threads", [1, 0])
+def test_psort_u64_and_f32(oracle, threads):
+    """oracle_psort_u64 against numpy's sort; oracle_psort_f32 against the
+    single-thread total-order sort and an independent numpy key (negative
+    floats reversed, -0 before +0, NaNs by sign and payload)."""
+    rng = np.random.default_rng(41)
+    u = rng.integers(0, 2**64, 1 << 18, dtype=np.uint64)
+    u[::5] = 7
+    assert np.array_equal(oracle.psort(u, threads), np.sort(u))
+    bits = rng.integers(0, 2**32, 1 << 18, dtype=np.uint64).astype(np.uint32)
+    bits[:4] = [0x80000000, 0, 0x7FC00000, 0xFFC00000]
+    f = bits.view(np.float32)
+    got = oracle.psort(f, threads).view(np.uint32)
+    assert np.array_equal(got, oracle.sort(f).view(np.uint32))
+    key = np.where(bits >> 31 == 1, ~bits, bits | np.uint32(0x80000000)).astype(np.uint32)
+    assert np.array_equal(got, bits[np.argsort(key, kind="stable")])
+
+
+def test_time_sort_runs(oracle):
+    a = oracle.gen_i32(5, 1 << 16, 32)
+    secs = np.zeros(3)
+    assert oracle.lib.oracle_time_sort_i32_runs(a.ctypes.data, a.size, 1, 1, 3, secs.ctypes.data) == 0
+    assert (secs > 0).all() and (secs < 10).all()
