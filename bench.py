@@ -78,6 +78,8 @@ def parse():
     ap.add_argument("--kernel", type=int, default=None,
                     help="merge kernel: 0 wavefront tile (default), 1 minBlockHeap, 2 workgroup tile")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", choices=("sweep", "plain"), default="sweep",
+                    help="plain: only gmms_copy (the calibration kernel of the traffic counters)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-literal", action="store_true",
                     help="all 8192 segments (about 4e9 keys) instead of the 2^28-key prefix")
@@ -486,7 +488,7 @@ def other_configs(oracle, dev, stream):
     return out
 
 
-def copy_probe(dev, stream):
+def copy_probe(dev, stream, sweep=True):
     """Every variant of the stream-copy probe and torch's device copy over
     4 GiB (best of 5 each, HIP events on the stream)."""
     import torch
@@ -511,7 +513,7 @@ def copy_probe(dev, stream):
 
     rows = [{"kernel": "gmms_copy (1 load per lane, grid 32 per CU)", "GBps": best(lambda: gmms.copy(b, a, stream=stream))},
             {"kernel": "torch copy_", "GBps": best(lambda: b.copy_(a))}]
-    for unroll in (1, 2, 4, 8):
+    for unroll in (1, 2, 4, 8) if sweep else ():
         for nt in (False, True):
             for bpc in (0, 8, 32):
                 rows.append({"kernel": "gmms_copy_probe", "unroll": unroll, "nontemporal": nt, "blocks_per_cu": bpc,
@@ -649,7 +651,7 @@ def main():
             per_round.append(round(sum(vals) / len(vals), 4))
 
     # measured stream-copy peak: the best of every probe variant
-    copy_rows, copy_ok = copy_probe(dev, stream)
+    copy_rows, copy_ok = copy_probe(dev, stream, args.copy_probe == "sweep")
     copy_peak = max(r["GBps"] for r in copy_rows)
     log(f"copy probe: best {copy_peak} GB/s")
 
