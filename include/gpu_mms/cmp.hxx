@@ -11,13 +11,19 @@
  *   host_cmp<T>  the same order for host-side checks
  *   greater<T>, host_greater<T>  the reversed order, a > b (descending)
  *
- * The GPU never calls these functions: libgmms.so runs the built-in order of
- * the element type (integers by '<', floats by IEEE total order, pairs by
- * (key, value)), or its exact reverse. multimergesort.hxx accepts cmp<DATATYPE>
- * and greater<DATATYPE> as its comparator argument and rejects any other at
- * compile time, so a caller can never get an order it did not ask for.
- * Unlike the reference, MAXVAL is not a sentinel: keys equal to MAXVAL sort
- * like any other key.
+ * cmp<DATATYPE> and greater<DATATYPE> name libgmms.so's built-in orders (integers
+ * by '<', floats by IEEE total order, pairs by (key, value), or the exact
+ * reverse); the GPU never calls them. Any other order -- the reference's
+ * "edit this to be whatever comparison function is needed" -- is a
+ * comparator of the caller's own, a __host__ __device__ function
+ * int f(DATATYPE a, DATATYPE b) that is nonzero when a strictly precedes b,
+ * passed as multimergesort<DATATYPE, f>: include multimergesort_custom.hxx
+ * and compile with hipcc, and the header instantiates its kernels with it
+ * (stable: the result equals std::stable_sort with f). Through
+ * multimergesort.hxx alone such a comparator fails to compile, so a caller
+ * never silently gets an order it did not ask for. Editing cmp below does not
+ * change the library's order. Unlike the reference, MAXVAL is not a
+ * sentinel: keys equal to MAXVAL sort like any other key.
  */
 #ifndef GPU_MMS_CMP_HXX
 #define GPU_MMS_CMP_HXX

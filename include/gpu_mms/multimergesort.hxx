@@ -75,16 +75,36 @@ inline gmms_params drop_in_params(int threadBlocks, int descending = 0) {
 /* Status of the calling thread's last multimergesort call. */
 inline int multimergesort_status() { return gpu_mms::last_status_storage(); }
 
+#ifdef GPU_MMS_CUSTOM_ORDERS
+namespace gpu_mms {
+// defined in multimergesort_custom.hxx (hipcc): the sort under a caller order
+template <class T, int (*F)(T, T)>
+T* custom_order_sort(T* input, T* output, int64_t N);
+}  // namespace gpu_mms
+#endif
+
 template <typename T, fptr_t f>
 T* multimergesort(T* input, T* output, T* h_data, int threadBlocks, int64_t N) {
   static_assert(std::is_same<T, DATATYPE>::value, "T must be DATATYPE (cmp.hxx)");
+#ifdef GPU_MMS_CUSTOM_ORDERS
+  if constexpr (gpu_mms::order_of<f>::value < 0) {
+    // a caller-defined order: the header-only kernels of multimergesort_custom.hxx
+    (void)h_data;
+    (void)threadBlocks;
+    return gpu_mms::custom_order_sort<T, f>(input, output, N);
+  } else {
+#else
   static_assert(gpu_mms::order_of<f>::value >= 0,
-                "libgmms.so runs cmp<DATATYPE> (a < b) or greater<DATATYPE> (a > b)");
+                "libgmms.so runs cmp<DATATYPE> (a < b) or greater<DATATYPE> (a > b); for another order "
+                "include gpu_mms/multimergesort_custom.hxx instead and compile with hipcc");
+  {
+#endif
   void* result = nullptr;
   const gmms_params p = gpu_mms::drop_in_params(threadBlocks, gpu_mms::order_of<f>::value);
   const int st = gmms_multimergesort_params(gpu_mms::dtype_of<T>::value, input, output, h_data, &p, N, &result);
   gpu_mms::last_status_storage(st);
   return st == GMMS_SUCCESS ? static_cast<T*>(result) : nullptr;
+  }
 }
 
 template <typename T, fptr_t f>

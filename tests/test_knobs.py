@@ -64,3 +64,15 @@ def test_testmm_compiles_with_reference_knobs():
         r = subprocess.run([CXX, "-std=c++17", "-fsyntax-only", f"-I{INC}", "-I/opt/rocm/include",
                             "-D__HIP_PLATFORM_AMD__", *defs, TESTMM], capture_output=True, text=True)
         assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.skipif(CXX is None, reason="g++ missing")
+def test_custom_comparator_without_custom_header_is_refused(tmp_path):
+    """A caller order through multimergesort.hxx alone (host compiler, library
+    path) fails to compile with a message naming multimergesort_custom.hxx."""
+    src = tmp_path / "c.cpp"
+    src.write_text('#include "gpu_mms/multimergesort.hxx"\n'
+                   "int my_cmp(int a, int b) { return (a & 0xFF) < (b & 0xFF); }\n"
+                   "int* f(int* a, int* b, int n) { return multimergesort<int, my_cmp>(a, b, nullptr, n); }\n")
+    r = subprocess.run([CXX, "-std=c++17", "-fsyntax-only", f"-I{INC}", str(src)], capture_output=True, text=True)
+    assert r.returncode != 0 and "multimergesort_custom.hxx" in r.stderr
