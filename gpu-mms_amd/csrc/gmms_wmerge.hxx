@@ -165,6 +165,7 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
   constexpr int XS = VEC ? NREG : NREG + 1;  // transpose row stride (padded for one-word reads)
   constexpr int XW = WAVE * XS;           // transpose words per plane
   static_assert(!VEC || XC == 4, "the 16-byte transpose covers 16 slots per lane");
+  static_assert(!VEC || SAMP == 256, "a store row of 256 outputs holds one sample");
   uint32_t* win = my;                     // [list a][F] circular windows, W planes of WIN
   uint32_t* xp = win + WIN * W;           // transpose of the register tile, W planes of XW
 
@@ -346,9 +347,14 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
             if (f == 4 * WAVE && nb != 0) f = t * WAVE + __ffsll((long long)nb) - 1;
           }
 #pragma unroll
-          for (int t = 0; t < 4; ++t)
-            if (t * WAVE + lane < f)
-              gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + pos + t * WAVE + lane, g[t], a.flip);
+          for (int t = 0; t < 4; ++t) {
+            const int64_t o = obase + pos + t * WAVE + lane;
+            if (t * WAVE + lane < f) {
+              gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, o, g[t], a.flip);
+              if (a.samp_out_keys && (o & (SAMP - 1)) == 0)
+                gstore<DT, false>(a.samp_out_keys, a.samp_out_vals, o >> SAMP_SHIFT, g[t]);
+            }
+          }
           pos += f;
           if (f < 4 * WAVE) break;
         }
@@ -476,6 +482,38 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
       }
     }
     wait_dma();
+    // the next round's partition samples (every SAMP-th output, internal
+    // bits) of this step's outputs, from registers
+    if (a.samp_out_keys) {
+      if constexpr (VEC) {
+        // output obase + o with o = 256u + 4*lane + t: the multiples of SAMP
+        // (= 256) are o = s0 + 256u, all in lane s0/4, register 4u + s0%4
+        const int s0 = (int)((uint64_t)(-obase) & (uint64_t)(SAMP - 1));
+        if (lane == (s0 >> 2)) {
+          auto put = [&](auto tc) {
+            constexpr int T = decltype(tc)::value;
+#pragma unroll
+            for (int u = 0; u < XC; ++u)
+              if (256 * u + s0 < S)
+                gstore<DT, false>(a.samp_out_keys, a.samp_out_vals, (obase + 256 * u + s0) >> SAMP_SHIFT,
+                                  ov[4 * u + T]);
+          };
+          switch (s0 & 3) {  // wave-uniform
+            case 0: put(std::integral_constant<int, 0>{}); break;
+            case 1: put(std::integral_constant<int, 1>{}); break;
+            case 2: put(std::integral_constant<int, 2>{}); break;
+            default: put(std::integral_constant<int, 3>{}); break;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < NREG; ++u) {
+          const int o = u * WAVE + lane;
+          if (o < S && ((obase + o) & (SAMP - 1)) == 0)
+            gstore<DT, false>(a.samp_out_keys, a.samp_out_vals, (obase + o) >> SAMP_SHIFT, ov[u]);
+        }
+      }
+    }
     if constexpr (VEC) {
       // rows below S/256 are full (a wave-uniform test); in the partial row a
       // lane stores its four outputs at once, or one by one at the end
@@ -510,16 +548,6 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
     cur = cur2;
     head = head2;
     want = want2;
-  }
-  if (a.samp_out_keys) {
-    // the next round's partition samples (every SAMP-th output element) of
-    // this merger's range: read back from the output this wavefront has just
-    // written (its own stores, so they are visible to it in order)
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    const int64_t s1 = (obase + SAMP - 1) >> SAMP_SHIFT;
-    for (int64_t t = ((o0 + SAMP - 1) >> SAMP_SHIFT) + lane; t < s1; t += WAVE)
-      gstore<DT, false>(a.samp_out_keys, a.samp_out_vals, t,
-                        gload<DT, false>(a.out_keys, a.out_vals, t << SAMP_SHIFT));
   }
 }
 

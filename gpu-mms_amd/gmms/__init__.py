@@ -26,7 +26,8 @@ from dataclasses import dataclass
 from typing import Optional, Tuple
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libgmms.so")
+# the in-tree build, or another build of the same library named by GMMS_LIBRARY
+LIB_PATH = os.environ.get("GMMS_LIBRARY") or os.path.join(_HERE, "libgmms.so")
 HEADER_PATH = os.path.normpath(os.path.join(_HERE, "..", "..", "include", "gpu_mms", "gmms.h"))
 
 # dtype codes (gmms.h: gmms_dtype)
