@@ -115,6 +115,8 @@ struct gmms_server {
   std::condition_variable cv;
   std::deque<std::shared_ptr<Request>> pending;
   std::unordered_map<uint64_t, std::shared_ptr<Request>> live;
+  std::vector<hipEvent_t> free_events;  // ordering events of finished requests, for reuse
+  std::mutex ev_mu;                     // guards free_events (taken without mu)
   std::vector<std::unique_ptr<Batch>> inflight;
   uint64_t next_id = 1;
   int64_t next_batch = 0;
@@ -367,7 +369,10 @@ void scheduler(gmms_server* S) {
     if (!dead.empty()) {
       lk.unlock();
       for (gmms_seg_plan* P : dead) gmms_seg_plan_destroy(P);
-      for (hipEvent_t e : dead_ev) (void)hipEventDestroy(e);
+      if (!dead_ev.empty()) {  // their batches have finished: reusable
+        std::lock_guard<std::mutex> g(S->ev_mu);
+        S->free_events.insert(S->free_events.end(), dead_ev.begin(), dead_ev.end());
+      }
       lk.lock();
       continue;  // re-check: the state may have changed meanwhile
     }
@@ -499,7 +504,14 @@ int gmms_server_submit_after(gmms_server* S, int dtype, const void* keys, const 
     }
   }
   if (n > 0 && ordered) {
-    HIP_OK(hipEventCreateWithFlags(&r->after, hipEventDisableTiming));
+    {
+      std::lock_guard<std::mutex> g(S->ev_mu);
+      if (!S->free_events.empty()) {
+        r->after = S->free_events.back();
+        S->free_events.pop_back();
+      }
+    }
+    if (!r->after) HIP_OK(hipEventCreateWithFlags(&r->after, hipEventDisableTiming));
     const hipError_t e = hipEventRecord(r->after, (hipStream_t)stream);
     if (e != hipSuccess) {
       (void)hipEventDestroy(r->after);
@@ -571,6 +583,7 @@ void gmms_server_destroy(gmms_server* S) {
     S->cv.notify_all();
   }
   if (S->sched.joinable()) S->sched.join();
+  for (hipEvent_t e : S->free_events) (void)hipEventDestroy(e);
   for (hipStream_t s : S->streams) (void)hipStreamDestroy(s);
   for (void* p : S->staging)
     if (p) (void)hipHostFree(p);
