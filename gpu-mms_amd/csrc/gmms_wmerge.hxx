@@ -99,17 +99,18 @@ __device__ __forceinline__ void frontier_min(E x, int k, int Kp, E& y, int& ky) 
   y = from_words<E>(w);
   ky = __builtin_amdgcn_readfirstlane(k);
 }
-// 32-bit elements: one 64-bit key (element << 7 | lane), neutral all ones
+// 32-bit elements: a 32-bit minimum (one DPP-operand v_min per level), then
+// the lowest candidate lane holding it by a ballot (no 64-bit compares: they
+// cost about ten 32-bit instructions each on gfx950)
 template <>
 __device__ __forceinline__ void frontier_min<E32>(E32 x, int k, int Kp, E32& y, int& ky) {
-  uint64_t key = k == WAVE ? ~0ull : (((uint64_t)x.k << 7) | (uint64_t)k);
-  // butterfly over lane groups of 2..Kp with the DPP / swizzle exchanges;
-  // lane 0 ends with the minimum
+  const bool cand = k != WAVE;
+  uint32_t m = cand ? x.k : 0xFFFFFFFFu;
   auto lvl = [&](auto dc) {
     constexpr int D = decltype(dc)::value;
     if (D < Kp) {
-      const uint64_t o = ((uint64_t)rot_lane<D>((uint32_t)(key >> 32)) << 32) | rot_lane<D>((uint32_t)key);
-      key = o < key ? o : key;
+      const uint32_t o = rot_lane<D>(m);
+      m = o < m ? o : m;
     }
   };
   lvl(std::integral_constant<int, 1>{});
@@ -118,10 +119,9 @@ __device__ __forceinline__ void frontier_min<E32>(E32 x, int k, int Kp, E32& y, 
   lvl(std::integral_constant<int, 8>{});
   lvl(std::integral_constant<int, 16>{});
   lvl(std::integral_constant<int, 32>{});
-  const uint64_t kk = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(key >> 32)) << 32) |
-                      (uint32_t)__builtin_amdgcn_readfirstlane((int)key);
-  ky = kk == ~0ull ? WAVE : (int)(kk & 127u);
-  y.k = (uint32_t)(kk >> 7);
+  y.k = (uint32_t)__builtin_amdgcn_readfirstlane((int)m);
+  const unsigned long long at = __ballot(cand && x.k == y.k);  // candidate lanes are k == lane
+  ky = at ? __ffsll((long long)at) - 1 : WAVE;
 }
 
 // LDS DMA of element base + off into word planes dst[q * plane] (base and
@@ -147,6 +147,25 @@ __device__ __forceinline__ void dma_elem(const RoundArgs& a, int64_t base, uint3
                                        0);
     }
   }
+}
+
+// min(rem, F) for a non-negative 64-bit count, from its 32-bit halves: a
+// 64-bit vector compare costs about ten 32-bit instructions on gfx950, and
+// the empty asm keeps the compiler from folding the halves back into one.
+__device__ __forceinline__ int clamp_count(int64_t rem, int F) {
+  uint32_t hi = (uint32_t)((uint64_t)rem >> 32);
+  const uint32_t lo = (uint32_t)rem;
+  asm("" : "+v"(hi));
+  const uint32_t m = lo < (uint32_t)F ? lo : (uint32_t)F;
+  return hi ? F : (int)m;
+}
+// the same for a wave-uniform count (scalar instructions only)
+__device__ __forceinline__ int clamp_count_u(int64_t rem, int F) {
+  uint32_t hi = (uint32_t)((uint64_t)rem >> 32);
+  const uint32_t lo = (uint32_t)rem;
+  asm("" : "+s"(hi));
+  const uint32_t m = lo < (uint32_t)F ? lo : (uint32_t)F;
+  return hi ? F : (int)m;
 }
 
 // One merger, executed by one wavefront using the LDS region `my`.
@@ -184,6 +203,11 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
   const int64_t o0 = obase;  // this merger's output is [o0, final obase)
   const int64_t total = uniform64(wave_sum64(hi - lo));  // its output count
   int prio = -1;                                          // issue priority set last
+  // outputs written when the priority bands change: 8/16, 13/16, 15/16 of
+  // the total, as 32-bit counts (a heuristic only: saturated for mergers of
+  // 2^32 elements or more)
+  const uint64_t tot32 = total < 0xFFFFFFFFll ? (uint64_t)total : 0xFFFFFFFFull;
+  const uint32_t pb8 = (uint32_t)(tot32 / 2), pb13 = (uint32_t)(tot32 * 13 / 16), pb15 = (uint32_t)(tot32 * 15 / 16);
   // compact the non-empty lists, keeping list order
   const bool act = lane < K && hi > lo;
   const unsigned long long amask = __ballot(act);
@@ -254,7 +278,7 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
   // window state: list a's window holds its next `want` elements, logical
   // [0, want) at physical (head + i) & (F-1); cur = its first element
   int head = 0;
-  int want = live ? (int)(endp - cur < F ? endp - cur : F) : 0;
+  int want = live ? clamp_count(endp - cur, F) : 0;
   refill(cur, head, 0, want);
   wait_dma();
 
@@ -267,8 +291,8 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
     // towards the end (1/2, 5/16, 1/8, 1/16 of the output), where the order
     // among equal priorities decides who finishes last.
     {
-      const int64_t d16 = (obase - o0) * 16;
-      const int lvl = d16 < 8 * total ? 3 : d16 < 13 * total ? 2 : d16 < 15 * total ? 1 : 0;
+      const uint32_t done = (uint32_t)obase - (uint32_t)o0;
+      const int lvl = done < pb8 ? 3 : done < pb13 ? 2 : done < pb15 ? 1 : 0;
       if (lvl != prio) {
         prio = lvl;
         switch (lvl) {
@@ -333,10 +357,11 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
         for (;;) {
           E g[4];
           bool ok[4];
+          const int rel = clamp_count_u(lim - pos, 4 * WAVE);  // elements left in the run, up to a batch
 #pragma unroll
           for (int t = 0; t < 4; ++t) {
             const int64_t idx = pos + t * WAVE + lane;
-            const bool valid = idx < lim;
+            const bool valid = t * WAVE + lane < rel;
             g[t] = valid ? gload<DT, false>(a.in_keys, a.in_vals, src + idx) : Limits<E>::hi();
             ok[t] = valid && (kz == WAVE || !lt(z, g[t]));  // g <= z
           }
@@ -351,7 +376,7 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
             const int64_t o = obase + pos + t * WAVE + lane;
             if (t * WAVE + lane < f) {
               gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, o, g[t], a.flip);
-              if (a.samp_out_keys && (o & (SAMP - 1)) == 0)
+              if (a.samp_out_keys && ((uint32_t)o & (uint32_t)(SAMP - 1)) == 0)
                 gstore<DT, false>(a.samp_out_keys, a.samp_out_vals, o >> SAMP_SHIFT, g[t]);
             }
           }
@@ -372,7 +397,7 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
         c2 = 0;
       }
       obase += glen;
-      const int w2 = live ? (int)(endp - cur < F ? endp - cur : F) : 0;
+      const int w2 = live ? clamp_count(endp - cur, F) : 0;
       refill(cur, head, c2, w2);
       want = w2;
       wait_dma();
@@ -394,7 +419,7 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
     const int64_t cur2 = cur + sk;
     const int head2 = (head + sk) & (F - 1);
     const int cnt2 = want - sk;
-    const int want2 = live ? (int)(endp - cur2 < F ? endp - cur2 : F) : 0;
+    const int want2 = live ? clamp_count(endp - cur2, F) : 0;
     // outputs (o < S) in the store layout above
     E ov[NREG];
     if (Kp == 1) {
@@ -509,7 +534,7 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
 #pragma unroll
         for (int u = 0; u < NREG; ++u) {
           const int o = u * WAVE + lane;
-          if (o < S && ((obase + o) & (SAMP - 1)) == 0)
+          if (o < S && (((uint32_t)obase + (uint32_t)o) & (uint32_t)(SAMP - 1)) == 0)
             gstore<DT, false>(a.samp_out_keys, a.samp_out_vals, (obase + o) >> SAMP_SHIFT, ov[u]);
         }
       }
