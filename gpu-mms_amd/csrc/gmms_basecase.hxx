@@ -18,6 +18,7 @@
 // Loads are fully coalesced (lane + 64*r); the sorted tile goes back to
 // memory through a bank-padded LDS transpose so the stores are coalesced too.
 #pragma once
+#include <type_traits>
 #include "gmms_elem.hxx"
 
 namespace gmms {
@@ -68,9 +69,75 @@ __device__ __forceinline__ void flip_lanes(E (&v)[N], int lane) {
   for (int r = 0; r < N; ++r) v[r] = lane_ce(v[r], p[r], low, hsel);
 }
 
+// 32-bit keys, half-cleaner at lane distance 4, four registers: DPP has no
+// xor-4 pattern, but a lane pair at distance 4 lies in one row, the lower
+// lane in bank 0 or 2 (lanes 0-3, 8-11 of the row), the upper in bank 1 or 3.
+// The lower lanes take min(x, lane + 4) by row_ror:12, the upper lanes
+// max(x, lane - 4) by row_ror:4, each instruction writing only its banks:
+// two VALU instructions per register and no LDS round trip. The s_nop
+// before covers the DPP read of registers written just before the block,
+// the one after a DPP read of its last result just after it.
+__device__ __forceinline__ void clean4_u32x4(uint32_t& a, uint32_t& b, uint32_t& c, uint32_t& d) {
+  uint32_t ta, tb, tc, td;
+  asm("s_nop 1\n\t"
+      "v_min_u32_dpp %0, %4, %4 row_ror:12 row_mask:0xf bank_mask:0x5\n\t"
+      "v_min_u32_dpp %1, %5, %5 row_ror:12 row_mask:0xf bank_mask:0x5\n\t"
+      "v_min_u32_dpp %2, %6, %6 row_ror:12 row_mask:0xf bank_mask:0x5\n\t"
+      "v_min_u32_dpp %3, %7, %7 row_ror:12 row_mask:0xf bank_mask:0x5\n\t"
+      "v_max_u32_dpp %0, %4, %4 row_ror:4 row_mask:0xf bank_mask:0xa\n\t"
+      "v_max_u32_dpp %1, %5, %5 row_ror:4 row_mask:0xf bank_mask:0xa\n\t"
+      "v_max_u32_dpp %2, %6, %6 row_ror:4 row_mask:0xf bank_mask:0xa\n\t"
+      "v_max_u32_dpp %3, %7, %7 row_ror:4 row_mask:0xf bank_mask:0xa\n\t"
+      "s_nop 1"
+      : "=&v"(ta), "=&v"(tb), "=&v"(tc), "=&v"(td)
+      : "v"(a), "v"(b), "v"(c), "v"(d));
+  a = ta;
+  b = tb;
+  c = tc;
+  d = td;
+}
+
+// 32-bit keys, half-cleaner at lane distance 16 or 32 on registers a, b:
+// v_permlane16_swap (32_swap) exchanges the odd rows of 16 (the upper 32
+// lanes) of a with the even rows (the lower 32 lanes) of b, so that every
+// lower lane's element and its upper partner meet in one lane, the lower one
+// in a; min and max there, and the same swap puts them back. Four VALU
+// instructions per two registers and no LDS round trip.
+template <int D>
+__device__ __forceinline__ void clean_rows_u32x2(uint32_t& a, uint32_t& b) {
+  static_assert(D == 16 || D == 32, "row swaps");
+  const auto s = D == 16 ? __builtin_amdgcn_permlane16_swap(a, b, false, false)
+                         : __builtin_amdgcn_permlane32_swap(a, b, false, false);
+  const uint32_t lo = s[0] < s[1] ? s[0] : s[1], hi = s[0] < s[1] ? s[1] : s[0];
+  const auto t = D == 16 ? __builtin_amdgcn_permlane16_swap(lo, hi, false, false)
+                         : __builtin_amdgcn_permlane32_swap(lo, hi, false, false);
+  a = t[0];
+  b = t[1];
+}
+
+// How the cross-lane stages of a network exchange values:
+//   XC_LDS   DPP where a pattern exists, ds_swizzle / ds_bpermute otherwise:
+//            the fewest VALU instructions, for kernels bound by VALU issue
+//            (the base case);
+//   XC_NOLDS 32-bit keys use DPP, bank-masked DPP and row swaps only, one or
+//            two VALU instructions more per register and stage but no LDS
+//            round trip in the dependency chain, for kernels bound by the
+//            latency of that chain (the merge step).
+enum : int { XC_LDS = 0, XC_NOLDS = 1 };
+
 // half-cleaner across lanes at lane distance D
-template <int D, class E, int N>
+template <int D, int X = XC_LDS, class E, int N>
 __device__ __forceinline__ void clean_lanes(E (&v)[N], int lane) {
+  if constexpr (X == XC_NOLDS && D == 4 && std::is_same<E, E32>::value && N % 4 == 0) {
+#pragma unroll
+    for (int r = 0; r < N; r += 4) clean4_u32x4(v[r].k, v[r + 1].k, v[r + 2].k, v[r + 3].k);
+    return;
+  }
+  if constexpr (X == XC_NOLDS && (D == 16 || D == 32) && std::is_same<E, E32>::value && N % 2 == 0) {
+#pragma unroll
+    for (int r = 0; r < N; r += 2) clean_rows_u32x2<D>(v[r].k, v[r + 1].k);
+    return;
+  }
   const bool low = (lane & D) == 0;
   const uint32_t hsel = low ? 0u : 0xFFFFFFFFu;
 #pragma unroll
@@ -80,20 +147,20 @@ __device__ __forceinline__ void clean_lanes(E (&v)[N], int lane) {
   }
 }
 
-template <int B, class E, int N>
+template <int B, int X = XC_LDS, class E, int N>
 __device__ __forceinline__ void lane_cleaners(E (&v)[N], int lane) {
-  if constexpr (B >= 6) clean_lanes<32>(v, lane);
-  if constexpr (B >= 5) clean_lanes<16>(v, lane);
-  if constexpr (B >= 4) clean_lanes<8>(v, lane);
-  if constexpr (B >= 3) clean_lanes<4>(v, lane);
-  if constexpr (B >= 2) clean_lanes<2>(v, lane);
-  if constexpr (B >= 1) clean_lanes<1>(v, lane);
+  if constexpr (B >= 6) clean_lanes<32, X>(v, lane);
+  if constexpr (B >= 5) clean_lanes<16, X>(v, lane);
+  if constexpr (B >= 4) clean_lanes<8, X>(v, lane);
+  if constexpr (B >= 3) clean_lanes<4, X>(v, lane);
+  if constexpr (B >= 2) clean_lanes<2, X>(v, lane);
+  if constexpr (B >= 1) clean_lanes<1, X>(v, lane);
 }
 
-template <int B, class E, int N>
+template <int B, int X = XC_LDS, class E, int N>
 __device__ __forceinline__ void merge_level(E (&v)[N], int lane) {
   flip_lanes<B>(v, lane);
-  lane_cleaners<B>(v, lane);
+  lane_cleaners<B, X>(v, lane);
   clean_regs(v);
 }
 

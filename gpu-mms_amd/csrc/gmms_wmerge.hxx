@@ -35,6 +35,17 @@ constexpr int WM_WAVES = 4;  // mergers per workgroup
 __host__ __device__ inline size_t wmerge_lds_words_per_wave(int W, int WIN) {
   return W == 1 ? (size_t)2 * WIN : (size_t)WIN * W + (size_t)WAVE * (WIN / WAVE + 1) * W;
 }
+// Workgroup LDS layout: the windows of all its mergers first (merger wv's W
+// planes of WIN words at wv * W * WIN), then their transposes. Every window
+// plane then starts at a multiple of 4 * WIN bytes, so a window of F <= WIN
+// words starts at a multiple of 4F bytes and a wrapped slot offset can be
+// OR-ed into its address.
+__device__ __forceinline__ uint32_t* wmerge_windows(uint32_t* lds, int wv, int W, int WIN) {
+  return lds + (size_t)wv * W * WIN;
+}
+__device__ __forceinline__ uint32_t* wmerge_transpose(uint32_t* lds, int wv, int W, int WIN) {
+  return lds + (size_t)WM_WAVES * W * WIN + (size_t)wv * (wmerge_lds_words_per_wave(W, WIN) - (size_t)W * WIN);
+}
 
 __device__ __forceinline__ void wave_lds_sync() {
   // LDS instructions of one wavefront execute in order; this keeps the
@@ -49,6 +60,21 @@ __device__ __forceinline__ E lget(const uint32_t* base, int cap, int idx) {
   uint32_t w[E::W];
 #pragma unroll
   for (int q = 0; q < E::W; ++q) w[q] = base[q * cap + idx];
+  return from_words<E>(w);
+}
+// The same read at an LDS byte address (word plane q at ab + 4 * q * cap).
+// Window slots are addressed as ((position * 4) & (4F - 1)) | (LDS address of
+// the window), one add and one and-or per slot: every window starts at a
+// multiple of 4F bytes (the wavefront regions are multiples of 8 KB apart).
+typedef __attribute__((address_space(3))) const uint32_t* lds_cu32;
+__device__ __forceinline__ uint32_t lds_addr(const uint32_t* p) {
+  return (uint32_t)(size_t)(lds_cu32)p;
+}
+template <class E>
+__device__ __forceinline__ E lget_b(int cap, uint32_t ab) {
+  uint32_t w[E::W];
+#pragma unroll
+  for (int q = 0; q < E::W; ++q) w[q] = *(lds_cu32)(size_t)(ab + 4u * (uint32_t)(q * cap));
   return from_words<E>(w);
 }
 template <class E>
@@ -149,6 +175,26 @@ __device__ __forceinline__ void dma_elem(const RoundArgs& a, int64_t base, uint3
   }
 }
 
+// The same DMA with the key byte offset kb of a 32-bit-key element (4 bytes
+// per element) or of a 64-bit key (8 bytes; the value of a pair at kb / 2).
+template <int DT>
+__device__ __forceinline__ void dma_elem_b(const RoundArgs& a, int64_t base, uint32_t kb, uint32_t* dst, int plane) {
+  using E = typename DType<DT>::E;
+  typedef __attribute__((address_space(3))) void* lds_ptr;
+  if constexpr (E::W == 1) {
+    const char* k0 = (const char*)((const uint32_t*)a.in_keys + base);
+    __builtin_amdgcn_global_load_lds((const void*)(k0 + kb), (lds_ptr)dst, 4, 0, 0);
+  } else {
+    const char* kw = (const char*)((const uint64_t*)a.in_keys + base) + kb;
+    __builtin_amdgcn_global_load_lds((const void*)kw, (lds_ptr)dst, 4, 0, 0);
+    __builtin_amdgcn_global_load_lds((const void*)(kw + 4), (lds_ptr)(dst + plane), 4, 0, 0);
+    if constexpr (E::W == 3) {
+      const char* vb = (const char*)(a.in_vals + base) + (kb >> 1);
+      __builtin_amdgcn_global_load_lds((const void*)vb, (lds_ptr)(dst + 2 * plane), 4, 0, 0);
+    }
+  }
+}
+
 // min(rem, F) for a non-negative 64-bit count, from its 32-bit halves: a
 // 64-bit vector compare costs about ten 32-bit instructions on gfx950, and
 // the empty asm keeps the compiler from folding the halves back into one.
@@ -170,7 +216,8 @@ __device__ __forceinline__ int clamp_count_u(int64_t rem, int F) {
 
 // One merger, executed by one wavefront using the LDS region `my`.
 template <int DT, bool CONV_OUT, int WIN>
-__device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo& mi, int lane, uint32_t* my) {
+__device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo& mi, int lane, uint32_t* win,
+                                           uint32_t* xp) {
   using E = typename DType<DT>::E;
   constexpr int W = E::W;
   constexpr int CHUNKS = WIN / WAVE;
@@ -185,8 +232,8 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
   constexpr int XW = WAVE * XS;           // transpose words per plane
   static_assert(!VEC || XC == 4, "the 16-byte transpose covers 16 slots per lane");
   static_assert(!VEC || SAMP == 256, "a store row of 256 outputs holds one sample");
-  uint32_t* win = my;                     // [list a][F] circular windows, W planes of WIN
-  uint32_t* xp = win + WIN * W;           // transpose of the register tile, W planes of XW
+  // win: [list a][F] circular windows, W planes of WIN (wmerge_windows);
+  // xp: transpose of the register tile, W planes of XW (wmerge_transpose)
 
   const int K = mi.K;
   const int j = mi.j;
@@ -232,6 +279,12 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
   int Kp = 1;
   while (Kp < Ka) Kp <<= 1;
   const int F = WIN / Kp;  // window per active list (power of two >= WIN/64)
+  const int fsh = __builtin_ctz((unsigned)F);
+  const int tr = (lane * NREG) >> fsh;  // the run whose slots this lane holds in the register tile
+  // key bytes per element in the DMA offsets, and this lane's share of them
+  constexpr uint32_t KB = W == 1 ? 4u : 8u;
+  uint32_t laneb = (uint32_t)lane * KB;
+  asm("" : "+v"(laneb));  // opaque: keeps the refill offsets in key bytes (add, and, compare)
   // Window refill by LDS DMA: the logical range [c, w) of every active list
   // (physical slots (h + i) & (F-1) of its window) is loaded from global
   // memory straight into LDS; chunk u covers window slots u*64 .. u*64+63.
@@ -240,19 +293,29 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
   auto refill = [&](int64_t cu_, int h_, int c_, int w_) {
     if (F >= WAVE) {
       // a chunk lies inside one list: its state is a wave-uniform readlane
-      // of (head | cnt << 10 | want << 21) and of the 64-bit position
+      // of (head | cnt << 10 | want << 21) and of the 64-bit position.
+      // Slot u*64 + lane of list lr's window holds logical index
+      // i = (so + lane - h) & (F-1), so = (u*64) & (F-1); with
+      // j = (i - c) & (F-1) the slots to load are j < w - c, element
+      // cur + c + j (i in [c, w) exactly when j < w - c, as 0 <= c <= w <= F):
+      // one add, one and and one compare per chunk, in key bytes.
       const uint32_t pk = (uint32_t)h_ | ((uint32_t)c_ << 10) | ((uint32_t)w_ << 21);
+      // recomputed per call (scalar instructions) rather than kept across
+      // the loop, where they would be spilled and reloaded by readlanes
+      int ka = Ka, fs = fsh;
+      asm volatile("" : "+s"(ka), "+s"(fs));
 #pragma unroll
       for (int u = 0; u < CHUNKS; ++u) {
-        const int lr = (u * WAVE) / F;
-        if (lr < Ka) {
+        const int lr = (u * WAVE) >> fs;
+        if (lr < ka) {
           const uint32_t ps = (uint32_t)__builtin_amdgcn_readlane((int)pk, lr);
           const int hl = (int)(ps & 1023u), cl = (int)((ps >> 10) & 2047u), wl = (int)(ps >> 21);
           if (cl < wl) {
             const int64_t cu = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(cu_ >> 32), lr) << 32) |
                                          (uint32_t)__builtin_amdgcn_readlane((int)cu_, lr));
-            const int i = ((u * WAVE + lane) - hl) & (F - 1);
-            if (i >= cl && i < wl) dma_elem<DT>(a, cu, (uint32_t)i, win + u * WAVE, WIN);
+            const uint32_t sft = (uint32_t)(((u * WAVE) & ((1 << fs) - 1)) - hl - cl) * KB;
+            const uint32_t jb = (laneb + sft) & ((KB << fs) - 1u);
+            if (jb < (uint32_t)(wl - cl) * KB) dma_elem_b<DT>(a, cu + cl, jb, win + u * WAVE, WIN);
           }
         }
       }
@@ -304,11 +367,21 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
       }
     }
 
+    // the window state of this lane's tile run (see the register tile
+    // below), fetched first so that the permutes' LDS round trip overlaps
+    // the frontier and the searches
+    const int wr = __shfl(want, tr, WAVE);
+    const int hd = __shfl(head, tr, WAVE);
+
     // ---- frontier y = min over full windows of the last element, tag (elem, list)
     const bool full = live && want == F;
     E y;
     int ky;
-    frontier_min<E>(full ? lget<E>(win, WIN, lane * F + ((head + F - 1) & (F - 1))) : Limits<E>::hi(),
+    // this lane's window in bytes: lw4 | ((position * 4) & m4w)
+    const uint32_t lw4 = lds_addr(win) + (uint32_t)(lane * F) * 4u, m4w = (uint32_t)F * 4u - 1u;
+    uint32_t h4 = (uint32_t)head * 4u;
+    asm("" : "+v"(h4));
+    frontier_min<E>(full ? lget_b<E>(WIN, ((h4 + m4w - 3u) & m4w) | lw4) : Limits<E>::hi(),
                     full ? lane : WAVE, Kp, y, ky);
     // ---- elements each list takes: its window prefix <= y. Every element
     // still outside the windows is >= y (y is the smallest last element of a
@@ -329,7 +402,7 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
           int l = 0, h = want;
           while (l < h) {
             const int mid = (l + h) >> 1;
-            const E v = lget<E>(win, WIN, lane * F + ((head + mid) & (F - 1)));
+            const E v = lget_b<E>(WIN, ((h4 + 4u * (uint32_t)mid) & m4w) | lw4);
             const bool tk = !lt(y, v);  // v <= y
             l = tk ? mid + 1 : l;
             h = tk ? h : mid;
@@ -434,13 +507,16 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
       refill(cur2, head2, cnt2, want2);
     } else {
       // register tile: lane l holds slots l*NREG .. l*NREG+NREG-1, all in run r
-      const int r = (lane * NREG) / F;
+      const int r = tr;
       const int i0 = (lane * NREG) & (F - 1);
-      const int wr = __shfl(want, r, WAVE);
-      const int hd = __shfl(head, r, WAVE);
       E v[NREG];
+      // slot q: logical position hd + i0 + q of run r's window, which starts
+      // at word r*F (a multiple of F, so the wrapped offset ORs into it)
+      const uint32_t rb = lds_addr(win) + (uint32_t)(r * F) * 4u, m4 = (uint32_t)F * 4u - 1u;
+      uint32_t t4 = (uint32_t)(hd + i0) * 4u;
+      asm("" : "+v"(t4));  // keeps the per-slot address at one add and one and-or
 #pragma unroll
-      for (int q = 0; q < NREG; ++q) v[q] = lget<E>(win, WIN, r * F + ((hd + i0 + q) & (F - 1)));
+      for (int q = 0; q < NREG; ++q) v[q] = lget_b<E>(WIN, ((t4 + 4u * (uint32_t)q) & m4) | rb);
       // The untaken elements of a window are > y, so they sort behind the S
       // taken ones and may stay; only slots past a list's window contents
       // (a list near its end) need the filler.
@@ -453,12 +529,12 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
       // merge the Kp sorted blocks of F slots: levels from block F up to WIN
       const int b0 = __builtin_ctz((unsigned)(F / NREG));
       switch (b0) {
-        case 0: merge_level<0>(v, lane); [[fallthrough]];
-        case 1: merge_level<1>(v, lane); [[fallthrough]];
-        case 2: merge_level<2>(v, lane); [[fallthrough]];
-        case 3: merge_level<3>(v, lane); [[fallthrough]];
-        case 4: merge_level<4>(v, lane); [[fallthrough]];
-        default: merge_level<5>(v, lane);
+        case 0: merge_level<0, XC_NOLDS>(v, lane); [[fallthrough]];
+        case 1: merge_level<1, XC_NOLDS>(v, lane); [[fallthrough]];
+        case 2: merge_level<2, XC_NOLDS>(v, lane); [[fallthrough]];
+        case 3: merge_level<3, XC_NOLDS>(v, lane); [[fallthrough]];
+        case 4: merge_level<4, XC_NOLDS>(v, lane); [[fallthrough]];
+        default: merge_level<5, XC_NOLDS>(v, lane);
       }
       if constexpr (VEC) {
         // transpose through LDS in 16-byte chunks so that every lane stores
@@ -585,8 +661,8 @@ __global__ __launch_bounds__(WM_WAVES* WAVE, 4) void k_wmerge(RoundArgs a) {
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);  // wave-uniform
   const int64_t merger = (int64_t)blockIdx.x * WM_WAVES + wv;
   if (merger >= a.tasks * a.m) return;  // whole wavefront leaves; no barriers in this kernel
-  uint32_t* my = lds + (size_t)wv * wmerge_lds_words_per_wave(E::W, WIN);
-  wmerge_one<DT, CONV_OUT, WIN>(a, plain_merger(a, merger), lane, my);
+  wmerge_one<DT, CONV_OUT, WIN>(a, plain_merger(a, merger), lane, wmerge_windows(lds, wv, E::W, WIN),
+                                wmerge_transpose(lds, wv, E::W, WIN));
 }
 
 // Batched segments: a persistent launch; every wavefront takes merger
@@ -599,13 +675,14 @@ __global__ __launch_bounds__(WM_WAVES* WAVE, 4) void k_wmerge_persistent(RoundAr
   extern __shared__ uint32_t lds[];
   const int lane = threadIdx.x & (WAVE - 1);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);  // wave-uniform
-  uint32_t* my = lds + (size_t)wv * wmerge_lds_words_per_wave(E::W, WIN);
+  uint32_t* win = wmerge_windows(lds, wv, E::W, WIN);
+  uint32_t* xp = wmerge_transpose(lds, wv, E::W, WIN);
   for (;;) {
     unsigned int it = 0;
     if (lane == 0) it = atomicAdd(a.queue, 1u);
     it = (unsigned int)__shfl((int)it, 0, WAVE);
     if ((int64_t)it >= a.n_items) break;
-    wmerge_one<DT, CONV_OUT, WIN>(a, load_merger(a.items + it), lane, my);
+    wmerge_one<DT, CONV_OUT, WIN>(a, load_merger(a.items + it), lane, win, xp);
     wave_lds_sync();
   }
 }
