@@ -124,17 +124,19 @@ int launch_tmerge(RoundArgs& a, bool cout, hipStream_t s) {
 template <int DT, bool CONV, int WIN>
 int launch_wmerge_win(RoundArgs& a, hipStream_t s) {
   using E = typename DType<DT>::E;
-  const size_t lds = WM_WAVES * wmerge_lds_words_per_wave(E::W, WIN) * sizeof(uint32_t);
+  // one workgroup per CU's worth of resident mergers
+  const int nw = wm_resident_per_cu(E::W);
+  const size_t lds = ((size_t)nw * wmerge_lds_words_per_wave(E::W, WIN) + wmerge_peer_words(nw)) * sizeof(uint32_t);
   if (lds > LDS_LIMIT) return GMMS_ERROR_UNSUPPORTED;
   const int64_t mergers = a.tasks * a.m;
-  const dim3 grid((unsigned)((mergers + WM_WAVES - 1) / WM_WAVES));
+  const dim3 grid((unsigned)((mergers + nw - 1) / nw));
   static bool attr = false;
   if (!attr) {
     (void)hipFuncSetAttribute((const void*)k_wmerge<DT, CONV, WIN>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)LDS_LIMIT);
     attr = true;
   }
-  hipLaunchKernelGGL((k_wmerge<DT, CONV, WIN>), grid, dim3(WM_WAVES * WAVE), lds, s, a);
+  hipLaunchKernelGGL((k_wmerge<DT, CONV, WIN>), grid, dim3(nw * WAVE), lds, s, a);
   HIP_OK(hipGetLastError());
   return GMMS_SUCCESS;
 }

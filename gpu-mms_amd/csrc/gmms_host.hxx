@@ -40,12 +40,16 @@ constexpr int TM_F = 64;  // tile-merge window per list
 
 constexpr size_t LDS_LIMIT = 160 * 1024;  // gfx950 LDS per workgroup
 
-// resident wavefront mergers per CU for W-word elements: what the LDS holds,
-// at most 16 (4 wavefronts per SIMD: the kernel's register budget)
+// resident wavefront mergers per CU for W-word elements: what the LDS holds
+// (with the plain launch's progress table), at most WM_PLAIN_MAX = 16 (4
+// wavefronts per SIMD: the kernel's register budget), a multiple of 4; the
+// plain launch runs them as one workgroup
 inline int wm_resident_per_cu(int W) {
-  const size_t per_wg = WM_WAVES * wmerge_lds_words_per_wave(W, wm_win(W)) * sizeof(uint32_t);
-  const int wgs = (int)std::max<size_t>(1, LDS_LIMIT / per_wg);
-  return std::min(wgs * WM_WAVES, 16);
+  int nw = WM_PLAIN_MAX;
+  while (nw > 4 &&
+         ((size_t)nw * wmerge_lds_words_per_wave(W, wm_win(W)) + wmerge_peer_words(nw)) * sizeof(uint32_t) > LDS_LIMIT)
+    nw -= 4;
+  return nw;
 }
 
 inline int ilog2(int x) {

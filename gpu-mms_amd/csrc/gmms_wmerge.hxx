@@ -27,7 +27,11 @@
 
 namespace gmms {
 
-constexpr int WM_WAVES = 4;  // mergers per workgroup
+constexpr int WM_WAVES = 4;  // mergers per workgroup of the persistent (batched) launch
+// The plain launch puts all of a CU's resident mergers in one workgroup
+// (WM_PLAIN_MAX at most) so that the mergers sharing a SIMD can compare
+// their progress through LDS.
+constexpr int WM_PLAIN_MAX = 16;
 
 // LDS words per wavefront merger with a window budget of WIN elements: the
 // windows, and the transpose of the register tile for the store (bank-padded
@@ -43,9 +47,27 @@ __host__ __device__ inline size_t wmerge_lds_words_per_wave(int W, int WIN) {
 __device__ __forceinline__ uint32_t* wmerge_windows(uint32_t* lds, int wv, int W, int WIN) {
   return lds + (size_t)wv * W * WIN;
 }
-__device__ __forceinline__ uint32_t* wmerge_transpose(uint32_t* lds, int wv, int W, int WIN) {
-  return lds + (size_t)WM_WAVES * W * WIN + (size_t)wv * (wmerge_lds_words_per_wave(W, WIN) - (size_t)W * WIN);
+__device__ __forceinline__ uint32_t* wmerge_transpose(uint32_t* lds, int nw, int wv, int W, int WIN) {
+  return lds + (size_t)nw * W * WIN + (size_t)wv * (wmerge_lds_words_per_wave(W, WIN) - (size_t)W * WIN);
 }
+// LDS words after the nw mergers' regions of a plain launch: every
+// merger's SIMD, then four progress words per SIMD.
+__host__ __device__ inline size_t wmerge_peer_words(int nw) { return (size_t)nw + 16; }
+
+// Issue priority from the progress of the mergers that share this
+// wavefront's SIMD. A SIMD's instruction arbiter favours its older
+// wavefronts among equal priorities, so equal mergers sharing a SIMD would
+// otherwise finish tens of microseconds apart (measured: 76 us of a 710 us
+// round at 2^28 int32), the last ones alone on the SIMD and unable to hide
+// their latencies. The merger with the most outputs left gets the highest
+// priority, the one with the fewest the lowest, so the SIMD's mergers move
+// in step and finish together. Each SIMD has four progress words (outputs
+// left; all ones where no merger runs); a merger writes its own every step
+// and reads its SIMD's four without ordering, which a priority tolerates.
+struct WavePeers {
+  uint32_t grp;   // LDS byte address of the SIMD's four words
+  uint32_t mine;  // and of this merger's
+};
 
 __device__ __forceinline__ void wave_lds_sync() {
   // LDS instructions of one wavefront execute in order; this keeps the
@@ -215,9 +237,11 @@ __device__ __forceinline__ int clamp_count_u(int64_t rem, int F) {
 }
 
 // One merger, executed by one wavefront using the LDS region `my`.
-template <int DT, bool CONV_OUT, int WIN>
+// PEERS: issue priority by rank among the SIMD's mergers (wp); otherwise by
+// the fraction of this merger's output written.
+template <int DT, bool CONV_OUT, int WIN, bool PEERS>
 __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo& mi, int lane, uint32_t* win,
-                                           uint32_t* xp) {
+                                           uint32_t* xp, const WavePeers& wp) {
   using E = typename DType<DT>::E;
   constexpr int W = E::W;
   constexpr int CHUNKS = WIN / WAVE;
@@ -254,12 +278,14 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
   // the total, as 32-bit counts (a heuristic only: saturated for mergers of
   // 2^32 elements or more)
   const uint64_t tot32 = total < 0xFFFFFFFFll ? (uint64_t)total : 0xFFFFFFFFull;
-  const uint32_t pb8 = (uint32_t)(tot32 / 2), pb13 = (uint32_t)(tot32 * 13 / 16), pb15 = (uint32_t)(tot32 * 15 / 16);
+  const uint32_t pb8 = PEERS ? 0u : (uint32_t)(tot32 / 2), pb13 = PEERS ? 0u : (uint32_t)(tot32 * 13 / 16),
+                 pb15 = PEERS ? 0u : (uint32_t)(tot32 * 15 / 16);
   // compact the non-empty lists, keeping list order
   const bool act = lane < K && hi > lo;
   const unsigned long long amask = __ballot(act);
   const int Ka = __popcll(amask);
   if (Ka == 0) return;
+  const uint32_t tot_left = (uint32_t)tot32;  // outputs left, saturated at 2^32 - 1
   const int myidx = __popcll(amask & ((1ull << lane) - 1ull));
   // lane a takes the state of the a-th active list
   int src_lane = 0;
@@ -291,7 +317,29 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
   // Completion is awaited with vmcnt(0) later, so the loads of the next
   // step overlap this step's merge network.
   auto refill = [&](int64_t cu_, int h_, int c_, int w_) {
-    if (F >= WAVE) {
+    if (W == 1 && F == WAVE) {
+      // 32-bit keys, 64-slot windows (16 lists: the common case): chunk u
+      // is list u's window. Lane a packs ((h + c) & 63) * 4 | (w - c) * 4 << 12
+      // and the byte offset of element cur + c; slot s of window u then
+      // loads element byte jb = (4s - off) & 255 past it when jb < 4(w - c):
+      // three readlanes, a subtract, an and and a compare per chunk, and few
+      // scalar instructions (no per-chunk list index, skip test or field
+      // decode). Lanes past the active lists carry w = c: nothing loads.
+      const uint32_t pk =
+          ((uint32_t)((h_ + c_) & (WAVE - 1)) * 4u) | ((uint32_t)(w_ - c_) * 4u << 12);
+      const uint64_t eb = (uint64_t)(cu_ + c_) * 4u;
+      const char* kb = (const char*)a.in_keys;
+#pragma unroll
+      for (int u = 0; u < CHUNKS; ++u) {
+        const uint32_t ps = (uint32_t)__builtin_amdgcn_readlane((int)pk, u);
+        const uint64_t e = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(eb >> 32), u) << 32) |
+                           (uint32_t)__builtin_amdgcn_readlane((int)eb, u);
+        const uint32_t jb = (laneb - (ps & 0xFFFu)) & (WAVE * 4u - 1u);
+        typedef __attribute__((address_space(3))) void* lds_ptr;
+        if (jb < (ps >> 12))
+          __builtin_amdgcn_global_load_lds((const void*)(kb + e + jb), (lds_ptr)(win + u * WAVE), 4, 0, 0);
+      }
+    } else if (F >= WAVE) {
       // a chunk lies inside one list: its state is a wave-uniform readlane
       // of (head | cnt << 10 | want << 21) and of the 64-bit position.
       // Slot u*64 + lane of list lr's window holds logical index
@@ -350,12 +398,28 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
     // Issue priority by progress: the SIMD's arbiter favours older
     // wavefronts, so mergers of equal size sharing a SIMD would otherwise
     // finish far apart (the last ones alone, unable to hide latency); the
-    // merger furthest behind gets the highest priority. The bands narrow
-    // towards the end (1/2, 5/16, 1/8, 1/16 of the output), where the order
-    // among equal priorities decides who finishes last.
+    // merger furthest behind gets the highest priority. Plain launch: by
+    // rank among the SIMD's mergers (WavePeers). Batched launch (mergers of
+    // other workgroups share the SIMD): by the fraction written, in bands
+    // that narrow towards the end (1/2, 5/16, 1/8, 1/16 of the output),
+    // where the order among equal priorities decides who finishes last.
     {
       const uint32_t done = (uint32_t)obase - (uint32_t)o0;
-      const int lvl = done < pb8 ? 3 : done < pb13 ? 2 : done < pb15 ? 1 : 0;
+      int lvl;
+      if constexpr (PEERS) {
+        // rank of this merger's outputs left among its SIMD's mergers
+        const uint32_t left = tot_left - done;
+        *(volatile __attribute__((address_space(3))) uint32_t*)(size_t)wp.mine = left;
+        lvl = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t o = (uint32_t)__builtin_amdgcn_readfirstlane(
+              (int)*(const volatile __attribute__((address_space(3))) uint32_t*)(size_t)(wp.grp + 4u * q));
+          lvl += o < left ? 1 : 0;
+        }
+      } else {
+        lvl = done < pb8 ? 3 : done < pb13 ? 2 : done < pb15 ? 1 : 0;
+      }
       if (lvl != prio) {
         prio = lvl;
         switch (lvl) {
@@ -650,19 +714,36 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
     head = head2;
     want = want2;
   }
+  // done: ranks below every merger still running
+  if constexpr (PEERS) *(volatile __attribute__((address_space(3))) uint32_t*)(size_t)wp.mine = 0u;
 }
 
-// Plain sort: one merger per wavefront over the uniform task grid.
+// Plain sort: one merger per wavefront over the uniform task grid; a
+// workgroup holds a CU's resident mergers (blockDim.x / 64 of them).
 template <int DT, bool CONV_OUT, int WIN>
-__global__ __launch_bounds__(WM_WAVES* WAVE, 4) void k_wmerge(RoundArgs a) {
+__global__ __launch_bounds__(WM_PLAIN_MAX* WAVE, 4) void k_wmerge(RoundArgs a) {
   using E = typename DType<DT>::E;
   extern __shared__ uint32_t lds[];
   const int lane = threadIdx.x & (WAVE - 1);
+  const int nw = (int)(blockDim.x / WAVE);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);  // wave-uniform
-  const int64_t merger = (int64_t)blockIdx.x * WM_WAVES + wv;
-  if (merger >= a.tasks * a.m) return;  // whole wavefront leaves; no barriers in this kernel
-  wmerge_one<DT, CONV_OUT, WIN>(a, plain_merger(a, merger), lane, wmerge_windows(lds, wv, E::W, WIN),
-                                wmerge_transpose(lds, wv, E::W, WIN));
+  const int64_t merger = (int64_t)blockIdx.x * nw + wv;
+  // every merger's SIMD (HW_ID bits 5:4); each merger's progress word is
+  // the k-th of its SIMD's four, k = the mergers of lower index on that SIMD
+  uint32_t* tab = lds + (size_t)nw * wmerge_lds_words_per_wave(E::W, WIN);
+  const int simd = (int)((__builtin_amdgcn_s_getreg(4 | (0 << 6) | (31 << 11)) >> 4) & 3u);
+  if (lane == 0) tab[wv] = (uint32_t)simd;
+  if (wv == 0 && lane < 16) tab[nw + lane] = 0xFFFFFFFFu;
+  __syncthreads();  // the workgroup's only barrier
+  int k = 0;
+  for (int w = 0; w < wv; ++w) k += (int)tab[w] == simd ? 1 : 0;
+  if (merger >= a.tasks * a.m) return;  // whole wavefront leaves (its progress word stays all ones)
+  const uint32_t grp = lds_addr(tab + nw + 4 * simd);
+  uint32_t* win = wmerge_windows(lds, wv, E::W, WIN);
+  uint32_t* xp = wmerge_transpose(lds, nw, wv, E::W, WIN);
+  // (k < 4 at 4 wavefronts per SIMD; were it not, two mergers would share a
+  // word, which costs only priority accuracy)
+  wmerge_one<DT, CONV_OUT, WIN, true>(a, plain_merger(a, merger), lane, win, xp, WavePeers{grp, grp + 4u * (k & 3)});
 }
 
 // Batched segments: a persistent launch; every wavefront takes merger
@@ -676,13 +757,14 @@ __global__ __launch_bounds__(WM_WAVES* WAVE, 4) void k_wmerge_persistent(RoundAr
   const int lane = threadIdx.x & (WAVE - 1);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);  // wave-uniform
   uint32_t* win = wmerge_windows(lds, wv, E::W, WIN);
-  uint32_t* xp = wmerge_transpose(lds, wv, E::W, WIN);
+  uint32_t* xp = wmerge_transpose(lds, WM_WAVES, wv, E::W, WIN);
+  const WavePeers wp{0u, 0u};
   for (;;) {
     unsigned int it = 0;
     if (lane == 0) it = atomicAdd(a.queue, 1u);
     it = (unsigned int)__shfl((int)it, 0, WAVE);
     if ((int64_t)it >= a.n_items) break;
-    wmerge_one<DT, CONV_OUT, WIN>(a, load_merger(a.items + it), lane, win, xp);
+    wmerge_one<DT, CONV_OUT, WIN, false>(a, load_merger(a.items + it), lane, win, xp, wp);
     wave_lds_sync();
   }
 }
