@@ -118,13 +118,17 @@ __device__ __forceinline__ uint32_t rot_lane(uint32_t x) {
   else return xor_lane<D>(x);
 }
 
+// Lanes without a candidate hold the maximum, so the levels inside a row
+// (D <= 8: DPP, no LDS) run whatever Kp is; only the levels that go through
+// the LDS unit (16, 32) are skipped for Kp <= 16 (one uniform branch, not a
+// test per level).
 template <class E>
 __device__ __forceinline__ void frontier_min(E x, int k, int Kp, E& y, int& ky) {
   // butterfly over lane groups of 2..Kp with the DPP / swizzle exchanges;
   // lane 0 ends with the minimum
   auto lvl = [&](auto dc) {
     constexpr int D = decltype(dc)::value;
-    if (D < Kp) {
+    {
       uint32_t w[E::W];
 #pragma unroll
       for (int r = 0; r < E::W; ++r) w[r] = rot_lane<D>(word(x, r));
@@ -139,8 +143,10 @@ __device__ __forceinline__ void frontier_min(E x, int k, int Kp, E& y, int& ky) 
   lvl(std::integral_constant<int, 2>{});
   lvl(std::integral_constant<int, 4>{});
   lvl(std::integral_constant<int, 8>{});
-  lvl(std::integral_constant<int, 16>{});
-  lvl(std::integral_constant<int, 32>{});
+  if (Kp > 16) {
+    lvl(std::integral_constant<int, 16>{});
+    if (Kp > 32) lvl(std::integral_constant<int, 32>{});
+  }
   uint32_t w[E::W];
 #pragma unroll
   for (int q = 0; q < E::W; ++q) w[q] = (uint32_t)__builtin_amdgcn_readfirstlane((int)word(x, q));
@@ -156,17 +162,17 @@ __device__ __forceinline__ void frontier_min<E32>(E32 x, int k, int Kp, E32& y, 
   uint32_t m = cand ? x.k : 0xFFFFFFFFu;
   auto lvl = [&](auto dc) {
     constexpr int D = decltype(dc)::value;
-    if (D < Kp) {
-      const uint32_t o = rot_lane<D>(m);
-      m = o < m ? o : m;
-    }
+    const uint32_t o = rot_lane<D>(m);
+    m = o < m ? o : m;
   };
   lvl(std::integral_constant<int, 1>{});
   lvl(std::integral_constant<int, 2>{});
   lvl(std::integral_constant<int, 4>{});
   lvl(std::integral_constant<int, 8>{});
-  lvl(std::integral_constant<int, 16>{});
-  lvl(std::integral_constant<int, 32>{});
+  if (Kp > 16) {
+    lvl(std::integral_constant<int, 16>{});
+    if (Kp > 32) lvl(std::integral_constant<int, 32>{});
+  }
   y.k = (uint32_t)__builtin_amdgcn_readfirstlane((int)m);
   const unsigned long long at = __ballot(cand && x.k == y.k);  // candidate lanes are k == lane
   ky = at ? __ffsll((long long)at) - 1 : WAVE;
@@ -407,26 +413,24 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
       const uint32_t done = (uint32_t)obase - (uint32_t)o0;
       int lvl;
       if constexpr (PEERS) {
-        // rank of this merger's outputs left among its SIMD's mergers
+        // rank of this merger's outputs left among its SIMD's mergers:
+        // lanes 0-3 read the four words, a ballot counts the smaller ones
         const uint32_t left = tot_left - done;
         *(volatile __attribute__((address_space(3))) uint32_t*)(size_t)wp.mine = left;
-        lvl = 0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const uint32_t o = (uint32_t)__builtin_amdgcn_readfirstlane(
-              (int)*(const volatile __attribute__((address_space(3))) uint32_t*)(size_t)(wp.grp + 4u * q));
-          lvl += o < left ? 1 : 0;
-        }
+        const uint32_t o =
+            *(const volatile __attribute__((address_space(3))) uint32_t*)(size_t)(wp.grp + 4u * (uint32_t)(lane & 3));
+        lvl = __popcll(__ballot(o < left) & 0xFull);
       } else {
         lvl = done < pb8 ? 3 : done < pb13 ? 2 : done < pb15 ? 1 : 0;
       }
       if (lvl != prio) {
         prio = lvl;
-        switch (lvl) {
-          case 3: __builtin_amdgcn_s_setprio(3); break;
-          case 2: __builtin_amdgcn_s_setprio(2); break;
-          case 1: __builtin_amdgcn_s_setprio(1); break;
-          default: __builtin_amdgcn_s_setprio(0); break;
+        if (lvl >= 2) {
+          if (lvl == 3) __builtin_amdgcn_s_setprio(3);
+          else __builtin_amdgcn_s_setprio(2);
+        } else {
+          if (lvl == 1) __builtin_amdgcn_s_setprio(1);
+          else __builtin_amdgcn_s_setprio(0);
         }
       }
     }
@@ -445,8 +449,8 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
     const uint32_t lw4 = lds_addr(win) + (uint32_t)(lane * F) * 4u, m4w = (uint32_t)F * 4u - 1u;
     uint32_t h4 = (uint32_t)head * 4u;
     asm("" : "+v"(h4));
-    frontier_min<E>(full ? lget_b<E>(WIN, ((h4 + m4w - 3u) & m4w) | lw4) : Limits<E>::hi(),
-                    full ? lane : WAVE, Kp, y, ky);
+    const E lastv = full ? lget_b<E>(WIN, ((h4 + m4w - 3u) & m4w) | lw4) : Limits<E>::hi();
+    frontier_min<E>(lastv, full ? lane : WAVE, Kp, y, ky);
     // ---- elements each list takes: its window prefix <= y. Every element
     // still outside the windows is >= y (y is the smallest last element of a
     // full window), so these S elements are the next S outputs. Equal
@@ -459,7 +463,31 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
     if (ky == WAVE) {
       sk = want;  // no full window: everything left is in the windows
     } else {
-      if (live && want > 0) {
+      if (F == WAVE) {
+        // 64-slot windows: binary lifting, six dependent reads and no loop
+        // control. pos ends as the count of window elements <= y below 64;
+        // a full window whose last element is <= y (it then equals y, the
+        // smallest last element) takes all 64; a window holding fewer than
+        // 64 elements (a list's last ones) bounds every step by its count
+        // (taken only when some lane has such a window).
+        if (live && want > 0) {
+          uint32_t pos = 0;
+          if (__ballot(want < F) == 0) {
+#pragma unroll
+            for (uint32_t b = WAVE / 2; b >= 1; b >>= 1) {
+              const E v = lget_b<E>(WIN, ((h4 + 4u * (pos + b - 1u)) & m4w) | lw4);
+              pos = lt(y, v) ? pos : pos + b;
+            }
+          } else {
+#pragma unroll
+            for (uint32_t b = WAVE / 2; b >= 1; b >>= 1) {
+              const E v = lget_b<E>(WIN, ((h4 + 4u * (pos + b - 1u)) & m4w) | lw4);
+              pos = (lt(y, v) || (int)(pos + b) > want) ? pos : pos + b;
+            }
+          }
+          sk = (lane == ky || (full && !lt(y, lastv))) ? F : (int)pos;
+        }
+      } else if (live && want > 0) {
         if (lane == ky) {
           sk = F;
         } else {
