@@ -371,4 +371,23 @@ __device__ __forceinline__ void gstore4(void* keys, uint32_t* vals, int64_t base
   }
 }
 
+// Two consecutive 32-bit elements e[0..1] to base + o, base + o + 1 with one
+// 8-byte store (element alignment suffices).
+template <int DT, bool CONV>
+__device__ __forceinline__ void gstore2(void* keys, int64_t base, int o, const typename DType<DT>::E* e,
+                                        uint32_t flip = 0u) {
+  using D = DType<DT>;
+  static_assert(D::E::W == 1, "32-bit keys");
+  using Wd = typename D::Word;
+  typedef Wd Wd2 __attribute__((ext_vector_type(2), aligned(sizeof(Wd))));
+  Wd2 x;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    Wd k = (Wd)e[t].k;
+    if (CONV) k = D::out(k ^ (Wd)(int32_t)flip);
+    x[t] = k;
+  }
+  *(Wd2*)(static_cast<Wd*>(keys) + base + o) = x;
+}
+
 }  // namespace gmms

@@ -416,9 +416,13 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
         // rank of this merger's outputs left among its SIMD's mergers:
         // lanes 0-3 read the four words, a ballot counts the smaller ones
         const uint32_t left = tot_left - done;
-        *(volatile __attribute__((address_space(3))) uint32_t*)(size_t)wp.mine = left;
+        // the addresses are rebuilt from scalars every step rather than held
+        // in vector registers across the loop (pairs use all 128)
+        uint32_t ga = wp.grp, ma = wp.mine;
+        asm volatile("" : "+s"(ga), "+s"(ma));
+        *(volatile __attribute__((address_space(3))) uint32_t*)(size_t)ma = left;
         const uint32_t o =
-            *(const volatile __attribute__((address_space(3))) uint32_t*)(size_t)(wp.grp + 4u * (uint32_t)(lane & 3));
+            *(const volatile __attribute__((address_space(3))) uint32_t*)(size_t)(ga + 4u * (uint32_t)(lane & 3));
         lvl = __popcll(__ballot(o < left) & 0xFull);
       } else {
         lvl = done < pb8 ? 3 : done < pb13 ? 2 : done < pb15 ? 1 : 0;
@@ -463,8 +467,8 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
     if (ky == WAVE) {
       sk = want;  // no full window: everything left is in the windows
     } else {
-      if (F == WAVE) {
-        // 64-slot windows: binary lifting, six dependent reads and no loop
+      if (W == 1 && F == WAVE) {
+        // 32-bit keys in 64-slot windows: binary lifting, six dependent reads and no loop
         // control. pos ends as the count of window elements <= y below 64;
         // a full window whose last element is <= y (it then equals y, the
         // smallest last element) takes all 64; a window holding fewer than
@@ -682,14 +686,15 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
         // output obase + o with o = 256u + 4*lane + t: the multiples of SAMP
         // (= 256) are o = s0 + 256u, all in lane s0/4, register 4u + s0%4
         const int s0 = (int)((uint64_t)(-obase) & (uint64_t)(SAMP - 1));
-        if (lane == (s0 >> 2)) {
+        const int ns = S > s0 ? (S - s0 + SAMP - 1) >> SAMP_SHIFT : 0;  // samples in this step (<= XC)
+        if (ns > 0 && lane == (s0 >> 2)) {
+          const int64_t si = (obase + s0) >> SAMP_SHIFT;  // index of the first
           auto put = [&](auto tc) {
             constexpr int T = decltype(tc)::value;
+            gstore<DT, false>(a.samp_out_keys, a.samp_out_vals, si, ov[T]);
 #pragma unroll
-            for (int u = 0; u < XC; ++u)
-              if (256 * u + s0 < S)
-                gstore<DT, false>(a.samp_out_keys, a.samp_out_vals, (obase + 256 * u + s0) >> SAMP_SHIFT,
-                                  ov[4 * u + T]);
+            for (int u = 1; u < XC; ++u)
+              if (u < ns) gstore<DT, false>(a.samp_out_keys, a.samp_out_vals, si + u, ov[4 * u + T]);
           };
           switch (s0 & 3) {  // wave-uniform
             case 0: put(std::integral_constant<int, 0>{}); break;
@@ -719,10 +724,18 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
         } else if (u == nfull) {
           if (o + 4 <= S) {
             gstore4<DT, CONV_OUT>(a.out_keys, a.out_vals, obase, o, &ov[4 * u], a.flip);
-          } else {
-#pragma unroll
-            for (int t = 0; t < 3; ++t)
-              if (o + t < S) gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + o + t, ov[4 * u + t], a.flip);
+          } else if (o < S) {
+            // the lane holding the last 1-3 outputs: an 8-byte store of the
+            // first two, then the remaining one
+            const int rem = S - o;
+            if (rem & 2) gstore2<DT, CONV_OUT>(a.out_keys, obase, o, &ov[4 * u], a.flip);
+            if (rem & 1) {
+              // a lane-mask select: a plain ?: over two ov[] entries becomes
+              // an indexed access that moves ov into scratch memory
+              E e;
+              e.k = mask_sel(__ballot(rem & 2), word(ov[4 * u + 2], 0), word(ov[4 * u], 0));
+              gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, obase + o + (rem & 2), e, a.flip);
+            }
           }
         }
       }
@@ -765,6 +778,7 @@ __global__ __launch_bounds__(WM_PLAIN_MAX* WAVE, 4) void k_wmerge(RoundArgs a) {
   __syncthreads();  // the workgroup's only barrier
   int k = 0;
   for (int w = 0; w < wv; ++w) k += (int)tab[w] == simd ? 1 : 0;
+  k = __builtin_amdgcn_readfirstlane(k);  // wave-uniform (every lane read the same words)
   if (merger >= a.tasks * a.m) return;  // whole wavefront leaves (its progress word stays all ones)
   const uint32_t grp = lds_addr(tab + nw + 4 * simd);
   uint32_t* win = wmerge_windows(lds, wv, E::W, WIN);
