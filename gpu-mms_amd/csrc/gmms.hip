@@ -29,6 +29,16 @@ int launch_basecase(void* keys, uint32_t* vals, int64_t n, bool cin, bool cout, 
   const int64_t tiles = (n + TILE - 1) / TILE;
   const dim3 grid((unsigned)((tiles + BASE_WAVES - 1) / BASE_WAVES));
   const dim3 block(BASE_WAVES * WAVE);
+  if constexpr (DType<DT>::E::W == 1) {
+    if (run4) {
+      // 32-bit keys: one wavefront per run of 4096 (64 keys per lane)
+      const dim3 rgrid((unsigned)(((n + RUN - 1) / RUN + BASE_WAVES - 1) / BASE_WAVES));
+      if (cout) hipLaunchKernelGGL((k_basecase_run<DT, true, true>), rgrid, block, 0, s, keys, n, flip);
+      else hipLaunchKernelGGL((k_basecase_run<DT, true, false>), rgrid, block, 0, s, keys, n, flip);
+      HIP_OK(hipGetLastError());
+      return GMMS_SUCCESS;
+    }
+  }
   if (run4) {
     // the sort's own base case: maps the caller's bits in, and out only if no round follows
     if (cout) hipLaunchKernelGGL((k_basecase<DT, true, true, true>), grid, block, 0, s, keys, vals, n, flip);

@@ -387,6 +387,95 @@ __global__ __launch_bounds__(BASE_WAVES * WAVE) void k_basecase(void* keys, uint
   tile_store<DT, CONV_OUT>(keys, vals, base, cnt, flip, v, xp + wave * E::W * XPOSE_WORDS, lane);
 }
 
+// ---------------------------------------------------------------------
+// Runs of 4096 32-bit keys sorted by one wavefront alone: 64 keys per lane
+// (element e in lane e/64, register e%64). The network is the same bitonic
+// sort in flip form, but with 6 of the 12 index bits in registers 57 of its
+// 78 stages are in-register compare-exchanges (one VALU instruction per
+// element) and 21 cross lanes, against 42 and 36 with 16 keys per lane and
+// four wavefronts per run, and no workgroup barrier or LDS exchange is
+// left. The in-lane sort of 64 keys is Batcher's odd-even merge sort (543
+// exchanges, against 672 for the bitonic stages it replaces). About 1360
+// VALU instructions per 1024 keys, against 1665.
+constexpr int RUN = 4096;
+constexpr int RPL = RUN / WAVE;  // keys per lane
+
+// mirror stage across lanes for many registers: partners fetched two at a
+// time (registers r and N-1-r exchange with the mirrored lane's N-1-r and
+// r), so no array of N partners is live
+template <int B, class E, int N>
+__device__ __forceinline__ void flip_lanes_pairs(E (&v)[N], int lane) {
+  constexpr int M = (2 << B) - 1;
+  const bool low = ((lane >> B) & 1) == 0;
+  const uint32_t hsel = low ? 0u : 0xFFFFFFFFu;
+#pragma unroll
+  for (int r = 0; r < N / 2; ++r) {
+    const E a = xlane<M>(v[N - 1 - r]);
+    const E b = xlane<M>(v[r]);
+    v[r] = lane_ce(v[r], a, low, hsel);
+    v[N - 1 - r] = lane_ce(v[N - 1 - r], b, low, hsel);
+  }
+}
+
+template <int DT, bool CONV_IN, bool CONV_OUT>
+__global__ __launch_bounds__(BASE_WAVES * WAVE) void k_basecase_run(void* keys, int64_t n, uint32_t flip) {
+  using E = typename DType<DT>::E;
+  static_assert(E::W == 1, "32-bit keys");
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+  const int64_t base = ((int64_t)blockIdx.x * BASE_WAVES + wave) * RUN;
+  if (base >= n) return;  // no barriers in this kernel
+  const int64_t rest = n - base;
+  const int cnt = rest < RUN ? (int)rest : RUN;
+
+  E v[RPL];
+  if (cnt == RUN) {
+    // registers 4k..4k+3 of lane l: elements 256k + 4l .. + 3 (16-byte loads;
+    // any placement works, the run is sorted as a whole)
+#pragma unroll
+    for (int k = 0; k < RPL / 4; ++k) gload4<DT, CONV_IN>(keys, base, 256 * k + 4 * lane, &v[4 * k], flip);
+  } else {
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) {
+      const int idx = r * WAVE + lane;
+      v[r] = idx < cnt ? gload<DT, CONV_IN>(keys, nullptr, base + idx, flip) : Limits<E>::hi();
+    }
+  }
+
+  oddeven_sort(v);
+  auto level = [&](auto bc) {
+    constexpr int B = decltype(bc)::value;
+    flip_lanes_pairs<B>(v, lane);
+    lane_cleaners<B>(v, lane);
+    clean_regs(v);
+  };
+  level(std::integral_constant<int, 0>{});
+  level(std::integral_constant<int, 1>{});
+  level(std::integral_constant<int, 2>{});
+  level(std::integral_constant<int, 3>{});
+  level(std::integral_constant<int, 4>{});
+  level(std::integral_constant<int, 5>{});
+
+  // lane l holds ranks 64l .. 64l + 63: sixteen 16-byte stores of
+  // consecutive ranks per lane (the partial run stores its first cnt ranks)
+  const int o = RPL * lane;
+  if (cnt == RUN) {
+#pragma unroll
+    for (int k = 0; k < RPL / 4; ++k) gstore4<DT, CONV_OUT>(keys, nullptr, base, o + 4 * k, &v[4 * k], flip);
+  } else {
+#pragma unroll
+    for (int k = 0; k < RPL / 4; ++k) {
+      if (o + 4 * k + 4 <= cnt) {
+        gstore4<DT, CONV_OUT>(keys, nullptr, base, o + 4 * k, &v[4 * k], flip);
+      } else {
+#pragma unroll
+        for (int t = 0; t < 3; ++t)
+          if (o + 4 * k + t < cnt) gstore<DT, CONV_OUT>(keys, nullptr, base + o + 4 * k + t, v[4 * k + t], flip);
+      }
+    }
+  }
+}
+
 // Batched segments: run descriptors instead of one array. A descriptor holds
 // `cnt` elements of one segment at `off` (1..TILE, or 1..4*TILE for the
 // workgroup-run variant); it is read from buffer 0 in the caller's bits and
