@@ -417,28 +417,24 @@ __device__ __forceinline__ void flip_lanes_pairs(E (&v)[N], int lane) {
   }
 }
 
+// One run of cnt (1..4096) 32-bit keys at src + base, sorted into
+// dst + base by the calling wavefront.
 template <int DT, bool CONV_IN, bool CONV_OUT>
-__global__ __launch_bounds__(BASE_WAVES * WAVE) void k_basecase_run(void* keys, int64_t n, uint32_t flip) {
+__device__ __forceinline__ void sort_run(const void* src, void* dst, int64_t base, int cnt, uint32_t flip,
+                                         int lane) {
   using E = typename DType<DT>::E;
   static_assert(E::W == 1, "32-bit keys");
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
-  const int64_t base = ((int64_t)blockIdx.x * BASE_WAVES + wave) * RUN;
-  if (base >= n) return;  // no barriers in this kernel
-  const int64_t rest = n - base;
-  const int cnt = rest < RUN ? (int)rest : RUN;
-
   E v[RPL];
   if (cnt == RUN) {
     // registers 4k..4k+3 of lane l: elements 256k + 4l .. + 3 (16-byte loads;
     // any placement works, the run is sorted as a whole)
 #pragma unroll
-    for (int k = 0; k < RPL / 4; ++k) gload4<DT, CONV_IN>(keys, base, 256 * k + 4 * lane, &v[4 * k], flip);
+    for (int k = 0; k < RPL / 4; ++k) gload4<DT, CONV_IN>(src, base, 256 * k + 4 * lane, &v[4 * k], flip);
   } else {
 #pragma unroll
     for (int r = 0; r < RPL; ++r) {
       const int idx = r * WAVE + lane;
-      v[r] = idx < cnt ? gload<DT, CONV_IN>(keys, nullptr, base + idx, flip) : Limits<E>::hi();
+      v[r] = idx < cnt ? gload<DT, CONV_IN>(src, nullptr, base + idx, flip) : Limits<E>::hi();
     }
   }
 
@@ -457,23 +453,34 @@ __global__ __launch_bounds__(BASE_WAVES * WAVE) void k_basecase_run(void* keys, 
   level(std::integral_constant<int, 5>{});
 
   // lane l holds ranks 64l .. 64l + 63: sixteen 16-byte stores of
-  // consecutive ranks per lane (the partial run stores its first cnt ranks)
+  // consecutive ranks per lane (a partial run stores its first cnt ranks)
   const int o = RPL * lane;
   if (cnt == RUN) {
 #pragma unroll
-    for (int k = 0; k < RPL / 4; ++k) gstore4<DT, CONV_OUT>(keys, nullptr, base, o + 4 * k, &v[4 * k], flip);
+    for (int k = 0; k < RPL / 4; ++k) gstore4<DT, CONV_OUT>(dst, nullptr, base, o + 4 * k, &v[4 * k], flip);
   } else {
 #pragma unroll
     for (int k = 0; k < RPL / 4; ++k) {
       if (o + 4 * k + 4 <= cnt) {
-        gstore4<DT, CONV_OUT>(keys, nullptr, base, o + 4 * k, &v[4 * k], flip);
+        gstore4<DT, CONV_OUT>(dst, nullptr, base, o + 4 * k, &v[4 * k], flip);
       } else {
 #pragma unroll
         for (int t = 0; t < 3; ++t)
-          if (o + 4 * k + t < cnt) gstore<DT, CONV_OUT>(keys, nullptr, base + o + 4 * k + t, v[4 * k + t], flip);
+          if (o + 4 * k + t < cnt) gstore<DT, CONV_OUT>(dst, nullptr, base + o + 4 * k + t, v[4 * k + t], flip);
       }
     }
   }
+}
+
+// The plain sort's base case for 32-bit keys: runs of 4096 in place.
+template <int DT, bool CONV_IN, bool CONV_OUT>
+__global__ __launch_bounds__(BASE_WAVES * WAVE) void k_basecase_run(void* keys, int64_t n, uint32_t flip) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+  const int64_t base = ((int64_t)blockIdx.x * BASE_WAVES + wave) * RUN;
+  if (base >= n) return;  // no barriers in this kernel
+  const int64_t rest = n - base;
+  sort_run<DT, CONV_IN, CONV_OUT>(keys, keys, base, rest < RUN ? (int)rest : RUN, flip, lane);
 }
 
 // Batched segments: run descriptors instead of one array. A descriptor holds
@@ -526,6 +533,22 @@ __global__ __launch_bounds__(BASE_WAVES* WAVE) void k_basecase_seg(void* k0, uin
   uint32_t* dv = (td.flags & 1) ? v1 : v0;
   if (td.flags & 2) tile_store<DT, true>(dk, dv, base, cnt, flip, v, my, lane);
   else tile_store<DT, false>(dk, dv, base, cnt, 0u, v, my, lane);
+}
+
+// Batched segments, 32-bit keys: one wavefront per run descriptor of up to
+// 4096 elements (as k_basecase_seg with RUN4, without the four-wavefront
+// exchanges).
+template <int DT>
+__global__ __launch_bounds__(BASE_WAVES* WAVE) void k_basecase_run_seg(void* k0, void* k1, const TileDesc* tiles,
+                                                                        int64_t n_tiles, uint32_t flip) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+  const int64_t t = (int64_t)blockIdx.x * BASE_WAVES + wave;
+  if (t >= n_tiles) return;  // no barriers in this kernel
+  const TileDesc td = tiles[t];
+  void* dk = (td.flags & 1) ? k1 : k0;
+  if (td.flags & 2) sort_run<DT, true, true>(k0, dk, td.off, td.cnt, flip, lane);
+  else sort_run<DT, true, false>(k0, dk, td.off, td.cnt, flip, lane);  // flip maps the load only
 }
 
 }  // namespace gmms

@@ -361,8 +361,14 @@ int seg_enqueue(const gmms_seg_plan* P, void* k0, void* k1, void* v0, void* v1, 
   {
     std::unique_ptr<StageScope> sc(timed ? new StageScope(GMMS_STAGE_BASECASE, P->elements, s) : nullptr);
     if (P->runs4 > 0) {
-      hipLaunchKernelGGL((k_basecase_seg<DT, true>), dim3((unsigned)P->runs4), dim3(BASE_WAVES * WAVE), 0, s, k0,
-                         (uint32_t*)v0, k1, (uint32_t*)v1, (const TileDesc*)tiles, P->runs4, P->flip);
+      if constexpr (DType<DT>::E::W == 1) {
+        // 32-bit keys: one wavefront per run descriptor
+        hipLaunchKernelGGL((k_basecase_run_seg<DT>), dim3((unsigned)((P->runs4 + BASE_WAVES - 1) / BASE_WAVES)),
+                           dim3(BASE_WAVES * WAVE), 0, s, k0, k1, (const TileDesc*)tiles, P->runs4, P->flip);
+      } else {
+        hipLaunchKernelGGL((k_basecase_seg<DT, true>), dim3((unsigned)P->runs4), dim3(BASE_WAVES * WAVE), 0, s, k0,
+                           (uint32_t*)v0, k1, (uint32_t*)v1, (const TileDesc*)tiles, P->runs4, P->flip);
+      }
       HIP_OK(hipGetLastError());
     }
     if (P->runs1 > 0) {
