@@ -131,24 +131,37 @@ int launch_tmerge(RoundArgs& a, bool cout, hipStream_t s) {
   return GMMS_SUCCESS;
 }
 
-template <int DT, bool CONV, int WIN>
-int launch_wmerge_win(RoundArgs& a, hipStream_t s) {
-  using E = typename DType<DT>::E;
-  // one workgroup per CU's worth of resident mergers
-  const int nw = wm_resident_per_cu(E::W);
-  const size_t lds = ((size_t)nw * wmerge_lds_words_per_wave(E::W, WIN) + wmerge_peer_words(nw)) * sizeof(uint32_t);
-  if (lds > LDS_LIMIT) return GMMS_ERROR_UNSUPPORTED;
+template <int DT, bool CONV, int WIN, bool PEERS>
+int launch_wmerge_kernel(RoundArgs& a, int nw, size_t lds, hipStream_t s) {
   const int64_t mergers = a.tasks * a.m;
   const dim3 grid((unsigned)((mergers + nw - 1) / nw));
   static bool attr = false;
   if (!attr) {
-    (void)hipFuncSetAttribute((const void*)k_wmerge<DT, CONV, WIN>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    (void)hipFuncSetAttribute((const void*)k_wmerge<DT, CONV, WIN, PEERS>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)LDS_LIMIT);
     attr = true;
   }
-  hipLaunchKernelGGL((k_wmerge<DT, CONV, WIN>), grid, dim3(nw * WAVE), lds, s, a);
+  hipLaunchKernelGGL((k_wmerge<DT, CONV, WIN, PEERS>), grid, dim3(nw * WAVE), lds, s, a);
   HIP_OK(hipGetLastError());
   return GMMS_SUCCESS;
+}
+
+template <int DT, bool CONV, int WIN>
+int launch_wmerge_win(RoundArgs& a, hipStream_t s) {
+  using E = typename DType<DT>::E;
+  const int per_cu = wm_resident_per_cu(E::W);
+  const int64_t mergers = a.tasks * a.m;
+  if (mergers <= (int64_t)per_cu * cu_count()) {
+    // every merger resident at once: one workgroup per CU's worth of mergers
+    const size_t lds = ((size_t)per_cu * wmerge_lds_words_per_wave(E::W, WIN) + wmerge_peer_words(per_cu)) *
+                       sizeof(uint32_t);
+    if (lds > LDS_LIMIT) return GMMS_ERROR_UNSUPPORTED;
+    return launch_wmerge_kernel<DT, CONV, WIN, true>(a, per_cu, lds, s);
+  }
+  // later mergers start as earlier ones finish: small workgroups
+  const size_t lds = (size_t)WM_WAVES * wmerge_lds_words_per_wave(E::W, WIN) * sizeof(uint32_t);
+  if (lds > LDS_LIMIT) return GMMS_ERROR_UNSUPPORTED;
+  return launch_wmerge_kernel<DT, CONV, WIN, false>(a, WM_WAVES, lds, s);
 }
 
 template <int DT>

@@ -781,9 +781,14 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
   if constexpr (PEERS) *(volatile __attribute__((address_space(3))) uint32_t*)(size_t)wp.mine = 0u;
 }
 
-// Plain sort: one merger per wavefront over the uniform task grid; a
-// workgroup holds a CU's resident mergers (blockDim.x / 64 of them).
-template <int DT, bool CONV_OUT, int WIN>
+// Plain sort: one merger per wavefront over the uniform task grid.
+// PEERS (the round's mergers all resident at once): a workgroup holds a
+// CU's resident mergers (blockDim.x / 64 of them), which rank their
+// progress against the others on their SIMD. Otherwise (more mergers than
+// resident slots: later ones start as earlier ones finish) workgroups of
+// WM_WAVES mergers, so that a finished workgroup frees its slots early,
+// with the progress bands.
+template <int DT, bool CONV_OUT, int WIN, bool PEERS>
 __global__ __launch_bounds__(WM_PLAIN_MAX* WAVE, 4) void k_wmerge(RoundArgs a) {
   using E = typename DType<DT>::E;
   extern __shared__ uint32_t lds[];
@@ -791,6 +796,13 @@ __global__ __launch_bounds__(WM_PLAIN_MAX* WAVE, 4) void k_wmerge(RoundArgs a) {
   const int nw = (int)(blockDim.x / WAVE);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);  // wave-uniform
   const int64_t merger = (int64_t)blockIdx.x * nw + wv;
+  uint32_t* win = wmerge_windows(lds, wv, E::W, WIN);
+  uint32_t* xp = wmerge_transpose(lds, nw, wv, E::W, WIN);
+  if constexpr (!PEERS) {
+    if (merger >= a.tasks * a.m) return;  // whole wavefront leaves; no barriers here
+    wmerge_one<DT, CONV_OUT, WIN, false>(a, plain_merger(a, merger), lane, win, xp, WavePeers{0u, 0u});
+    return;
+  }
   // every merger's SIMD (HW_ID bits 5:4); each merger's progress word is
   // the k-th of its SIMD's four, k = the mergers of lower index on that SIMD
   uint32_t* tab = lds + (size_t)nw * wmerge_lds_words_per_wave(E::W, WIN);
@@ -803,8 +815,6 @@ __global__ __launch_bounds__(WM_PLAIN_MAX* WAVE, 4) void k_wmerge(RoundArgs a) {
   k = __builtin_amdgcn_readfirstlane(k);  // wave-uniform (every lane read the same words)
   if (merger >= a.tasks * a.m) return;  // whole wavefront leaves (its progress word stays all ones)
   const uint32_t grp = lds_addr(tab + nw + 4 * simd);
-  uint32_t* win = wmerge_windows(lds, wv, E::W, WIN);
-  uint32_t* xp = wmerge_transpose(lds, nw, wv, E::W, WIN);
   // (k < 4 at 4 wavefronts per SIMD; were it not, two mergers would share a
   // word, which costs only priority accuracy)
   wmerge_one<DT, CONV_OUT, WIN, true>(a, plain_merger(a, merger), lane, win, xp, WavePeers{grp, grp + 4u * (k & 3)});
