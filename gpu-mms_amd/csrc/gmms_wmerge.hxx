@@ -223,30 +223,6 @@ __device__ __forceinline__ void dma_elem_b(const RoundArgs& a, int64_t base, uin
   }
 }
 
-// One chunk of a 64-slot window refill (32-bit keys): slot s (lane s)
-// loads the key at byte address g + jb, jb = (4s - ps) & 255, when
-// jb < ps >> 12, into LDS word m0 / 4 + s. The lane mask comes from a
-// v_cmpx (exec = the lanes that load) and exec is set back to all lanes
-// after the DMA, all inside one block, so the chunk needs no save, branch
-// and restore of the mask around the load: it must be called with every
-// lane active. The block writes LDS behind the compiler's back ("memory");
-// the caller waits for it with vmcnt(0) before reading the words.
-__device__ __forceinline__ void dma_chunk_u32(uint32_t ps, uint64_t g, uint32_t m0, uint32_t laneb) {
-  uint32_t jb, n;
-  asm volatile(
-      "v_sub_u32 %[jb], %[lb], %[ps]\n\t"
-      "v_and_b32 %[jb], 0xff, %[jb]\n\t"
-      "s_lshr_b32 %[n], %[ps], 12\n\t"
-      "v_cmpx_gt_u32 vcc, %[n], %[jb]\n\t"
-      "s_mov_b32 m0, %[m0]\n\t"
-      "s_nop 0\n\t"
-      "global_load_lds_dword %[jb], %[g]\n\t"
-      "s_mov_b64 exec, -1"
-      : [jb] "=&v"(jb), [n] "=&s"(n)
-      : [lb] "v"(laneb), [ps] "s"(ps), [g] "s"(g), [m0] "s"(m0)
-      : "memory", "vcc", "m0", "scc");
-}
-
 // min(rem, F) for a non-negative 64-bit count, from its 32-bit halves: a
 // 64-bit vector compare costs about ten 32-bit instructions on gfx950, and
 // the empty asm keeps the compiler from folding the halves back into one.
@@ -358,14 +334,16 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
       const uint32_t pk =
           ((uint32_t)((h_ + c_) & (WAVE - 1)) * 4u) | ((uint32_t)(w_ - c_) * 4u << 12);
       const uint64_t eb = (uint64_t)(cu_ + c_) * 4u;
-      const uint64_t kb = (uint64_t)(size_t)a.in_keys;
-      const uint32_t wb = lds_addr(win);
+      const char* kb = (const char*)a.in_keys;
 #pragma unroll
       for (int u = 0; u < CHUNKS; ++u) {
         const uint32_t ps = (uint32_t)__builtin_amdgcn_readlane((int)pk, u);
-        const uint32_t elo = (uint32_t)__builtin_amdgcn_readlane((int)eb, u);
-        const uint32_t ehi = (uint32_t)__builtin_amdgcn_readlane((int)(eb >> 32), u);
-        dma_chunk_u32(ps, kb + (((uint64_t)ehi << 32) | elo), wb + (uint32_t)u * (WAVE * 4u), laneb);
+        const uint64_t e = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(eb >> 32), u) << 32) |
+                           (uint32_t)__builtin_amdgcn_readlane((int)eb, u);
+        const uint32_t jb = (laneb - (ps & 0xFFFu)) & (WAVE * 4u - 1u);
+        typedef __attribute__((address_space(3))) void* lds_ptr;
+        if (jb < (ps >> 12))
+          __builtin_amdgcn_global_load_lds((const void*)(kb + e + jb), (lds_ptr)(win + u * WAVE), 4, 0, 0);
       }
     } else if (F >= WAVE) {
       // a chunk lies inside one list: its state is a wave-uniform readlane
