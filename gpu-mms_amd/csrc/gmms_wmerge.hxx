@@ -464,6 +464,7 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
     int sk = 0;
     bool gallop = false;
     int64_t glen = 0;
+    int gfill = -1;  // after a gallop: list ky's next window filled from registers (its count), or -1
     if (ky == WAVE) {
       sk = want;  // no full window: everything left is in the windows
     } else {
@@ -549,6 +550,23 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
                 gstore<DT, false>(a.samp_out_keys, a.samp_out_vals, o >> SAMP_SHIFT, g[t]);
             }
           }
+          if (f < 4 * WAVE) {
+            // the batch's elements past the run are list ky's next ones:
+            // when they cover its next window, they go into it from
+            // registers (slot i = element i, head 0) and the window needs
+            // no reload from memory
+            const int64_t rest = lim - pos - f;  // list ky's elements after the run
+            const int need = rest < (int64_t)F ? (int)rest : F;
+            if (F <= 4 * WAVE && rel - f >= need) {
+#pragma unroll
+              for (int t = 0; t < 4; ++t) {
+                const int i = t * WAVE + lane - f;
+                if (i >= 0 && i < need) lput<E>(win, WIN, ky * F + i, g[t]);
+              }
+              gfill = need;
+              wave_lds_sync();
+            }
+          }
           pos += f;
           if (f < 4 * WAVE) break;
         }
@@ -558,18 +576,21 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
     }
 
     if (gallop) {
-      // list ky restarts with an empty window; the others keep theirs
+      // list ky restarts with a window from its new position (filled above
+      // from the gallop's registers, or reloaded); the others keep theirs
       int c2 = want;
       if (lane == ky) {
         cur += glen;
         head = 0;
-        c2 = 0;
+        c2 = gfill < 0 ? 0 : gfill;
       }
       obase += glen;
       const int w2 = live ? clamp_count(endp - cur, F) : 0;
-      refill(cur, head, c2, w2);
       want = w2;
-      wait_dma();
+      if (gfill < 0) {
+        refill(cur, head, c2, w2);
+        wait_dma();
+      }
       continue;
     }
 
