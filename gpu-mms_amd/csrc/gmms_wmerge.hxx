@@ -333,17 +333,18 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
       // decode). Lanes past the active lists carry w = c: nothing loads.
       const uint32_t pk =
           ((uint32_t)((h_ + c_) & (WAVE - 1)) * 4u) | ((uint32_t)(w_ - c_) * 4u << 12);
-      const uint64_t eb = (uint64_t)(cu_ + c_) * 4u;
-      const char* kb = (const char*)a.in_keys;
+      // lane a's first element as a 64-bit address (one vector add for all
+      // lists): its two readlanes are the chunk's scalar base as they are
+      const uint64_t ep = (uint64_t)(size_t)a.in_keys + (uint64_t)(cu_ + c_) * 4u;
 #pragma unroll
       for (int u = 0; u < CHUNKS; ++u) {
         const uint32_t ps = (uint32_t)__builtin_amdgcn_readlane((int)pk, u);
-        const uint64_t e = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(eb >> 32), u) << 32) |
-                           (uint32_t)__builtin_amdgcn_readlane((int)eb, u);
+        const char* g = (const char*)(size_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(ep >> 32), u) << 32) |
+                                              (uint32_t)__builtin_amdgcn_readlane((int)ep, u));
         const uint32_t jb = (laneb - (ps & 0xFFFu)) & (WAVE * 4u - 1u);
         typedef __attribute__((address_space(3))) void* lds_ptr;
         if (jb < (ps >> 12))
-          __builtin_amdgcn_global_load_lds((const void*)(kb + e + jb), (lds_ptr)(win + u * WAVE), 4, 0, 0);
+          __builtin_amdgcn_global_load_lds((const void*)(g + jb), (lds_ptr)(win + u * WAVE), 4, 0, 0);
       }
     } else if (F >= WAVE) {
       // a chunk lies inside one list: its state is a wave-uniform readlane
