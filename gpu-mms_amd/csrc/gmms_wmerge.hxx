@@ -464,6 +464,7 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
     // instead of one list per step.
     int sk = 0;
     bool gallop = false;
+    int pre = 0;  // elements of one other list written ahead of a gallop
     int64_t glen = 0;
     int gfill = -1;  // after a gallop: list ky's next window filled from registers (its count), or -1
     if (ky == WAVE) {
@@ -510,8 +511,43 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
       }
       // gallop when no other list takes anything: list ky's run continues
       // past its window up to the smallest head z of the other lists
-      // (elements <= z)
-      gallop = __ballot(live && lane != ky && sk > 0) == 0;
+      // (elements <= z). Pairs also gallop when exactly one other list j
+      // takes, and all it takes precedes list ky's head: j's elements go
+      // out first, straight from its window, then ky gallops. That is a
+      // list handing over to the next one where equal keys are ordered by
+      // value across lists (each list's values in a range of their own, as
+      // when values are input positions): the step would otherwise merge a
+      // whole tile for a few outputs. (Keys without values have no such
+      // order among equal keys: every list takes its copies at once.)
+      const unsigned long long tk = __ballot(live && lane != ky && sk > 0);
+      if (E::HAS_VAL && tk != 0 && (tk & (tk - 1)) == 0) {
+        const int pj = __ffsll((long long)tk) - 1;
+        const int skj = __builtin_amdgcn_readlane(sk, pj);
+        if (skj <= WAVE && skj < __builtin_amdgcn_readlane(want, pj)) {
+          // list ky's head and list pj's last element taken (every lane
+          // reads the same two slots)
+          const int hj = __builtin_amdgcn_readlane(head, pj);
+          const E hk = lget<E>(win, WIN, ky * F + __builtin_amdgcn_readlane(head, ky));
+          const E lj = lget<E>(win, WIN, pj * F + ((hj + skj - 1) & (F - 1)));
+          if (!lt(hk, lj)) {
+            pre = skj;
+            if (lane < pre) {
+              const E x = lget<E>(win, WIN, pj * F + ((hj + lane) & (F - 1)));
+              const int64_t o = obase + lane;
+              gstore<DT, CONV_OUT>(a.out_keys, a.out_vals, o, x, a.flip);
+              if (a.samp_out_keys && ((uint32_t)o & (uint32_t)(SAMP - 1)) == 0)
+                gstore<DT, false>(a.samp_out_keys, a.samp_out_vals, o >> SAMP_SHIFT, x);
+            }
+            if (lane == pj) {
+              cur += pre;
+              head = (head + pre) & (F - 1);
+              want -= pre;
+            }
+            obase += pre;
+          }
+        }
+      }
+      gallop = tk == 0 || pre > 0;
       if (gallop) {
         E z;
         int kz;
@@ -588,7 +624,7 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
       obase += glen;
       const int w2 = live ? clamp_count(endp - cur, F) : 0;
       want = w2;
-      if (gfill < 0) {
+      if (gfill < 0 || pre > 0) {
         refill(cur, head, c2, w2);
         wait_dma();
       }

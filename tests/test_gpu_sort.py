@@ -147,6 +147,26 @@ def test_pairs(gmms, oracle, dev, dist):
     assert np.array_equal(_host(vs).view(np.uint32), wv)
 
 
+@pytest.mark.parametrize("distinct,n,mergers", [(64, 3_000_017, 0), (4096, 2_000_003, 101), (1 << 20, 1 << 21, 0)])
+def test_pairs_value_is_position_handover(gmms, oracle, dev, distinct, n, mergers):
+    """Zipf(1.1) keys over few or many distinct values, value = input
+    position: equal keys are ordered by value, so each list's copies of a
+    key follow the previous list's, and a merger hands over from one list
+    to the next (the elements one list takes before the next list's head
+    are written ahead of that list's gallop). Split tasks (mergers = 101)
+    start inside such runs."""
+    import torch
+
+    print(f"pairs handover distinct={distinct} n={n} mergers={mergers}", flush=True)
+    k, v = oracle.gen_pairs(77 + distinct, n, 3, 1.1, distinct)
+    p = gmms.make_params(K=16, mergers=mergers) if mergers else None
+    ks, vs = gmms.sort(torch.from_numpy(k.view(np.int64)).to(dev), torch.from_numpy(v.view(np.int32)).to(dev),
+                       params=p)
+    wk, wv = oracle.sort_pairs(k, v)
+    assert np.array_equal(_host(ks).view(np.uint64), wk)
+    assert np.array_equal(_host(vs).view(np.uint32), wv)
+
+
 @pytest.mark.parametrize("n,mergers", [(600_001, 37), (3_000_017, 101), (3_000_017, 3072)])
 def test_pairs_split_tasks(gmms, oracle, dev, n, mergers):
     """Tasks split over several mergers each (the merger count fills whole
