@@ -465,8 +465,6 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
     int sk = 0;
     bool gallop = false;
     int pre = 0;  // elements of one other list written ahead of a gallop
-    int64_t glen = 0;
-    int gfill = -1;  // after a gallop: list ky's next window filled from registers (its count), or -1
     if (ky == WAVE) {
       sk = want;  // no full window: everything left is in the windows
     } else {
@@ -548,18 +546,34 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
         }
       }
       gallop = tk == 0 || pre > 0;
-      if (gallop) {
+    }
+
+    if (gallop) {
+      // Gallops, chained for pairs: list gk's run from its first element up
+      // to the smallest head z of the other lists goes straight from input
+      // to output; then list kz (the one whose head is z) goes on the same
+      // way while its whole window precedes the next bound, without a step's
+      // frontier and searches in between (runs of equal keys ordered by
+      // value across lists pass from one list to the next like this).
+      int gk = ky;
+      for (bool first = true;; first = false) {
         E z;
         int kz;
-        const bool ne = live && want > 0 && lane != ky;
+        const bool ne = live && want > 0 && lane != gk;
         frontier_min<E>(ne ? lget<E>(win, WIN, lane * F + head) : Limits<E>::hi(), ne ? lane : WAVE, Kp, z, kz);
-        // list ky's run, from cur, that precedes z goes straight from input
+        if (!first && kz != WAVE) {
+          // list gk goes on only if its window's last element is <= z
+          const int hg = __builtin_amdgcn_readlane(head, gk), wg = __builtin_amdgcn_readlane(want, gk);
+          if (lt(z, lget<E>(win, WIN, gk * F + ((hg + wg - 1) & (F - 1))))) break;
+        }
+        int gfill = -1;  // list gk's next window filled from registers (its count), or -1
+        // list gk's run, from cur, that precedes z goes straight from input
         // to output: 4 elements per lane in flight, each batch checked
         // against z, up to the first element past z (the run is sorted, so
         // the check holds for a prefix) or the end of the list. One memory
         // round trip per 256 elements and none to find the end first.
-        const int64_t src = uniform64(__shfl(cur, ky, WAVE));
-        const int64_t lim = uniform64(__shfl(endp, ky, WAVE)) - src;
+        const int64_t src = uniform64(__shfl(cur, gk, WAVE));
+        const int64_t lim = uniform64(__shfl(endp, gk, WAVE)) - src;
         int64_t pos = 0;
         for (;;) {
           E g[4];
@@ -588,17 +602,17 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
             }
           }
           if (f < 4 * WAVE) {
-            // the batch's elements past the run are list ky's next ones:
+            // the batch's elements past the run are list gk's next ones:
             // when they cover its next window, they go into it from
             // registers (slot i = element i, head 0) and the window needs
             // no reload from memory
-            const int64_t rest = lim - pos - f;  // list ky's elements after the run
+            const int64_t rest = lim - pos - f;  // list gk's elements after the run
             const int need = rest < (int64_t)F ? (int)rest : F;
             if (F <= 4 * WAVE && rel - f >= need) {
 #pragma unroll
               for (int t = 0; t < 4; ++t) {
                 const int i = t * WAVE + lane - f;
-                if (i >= 0 && i < need) lput<E>(win, WIN, ky * F + i, g[t]);
+                if (i >= 0 && i < need) lput<E>(win, WIN, gk * F + i, g[t]);
               }
               gfill = need;
               wave_lds_sync();
@@ -607,26 +621,29 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
           pos += f;
           if (f < 4 * WAVE) break;
         }
-        glen = pos;
-        sk = 0;
-      }
-    }
-
-    if (gallop) {
-      // list ky restarts with a window from its new position (filled above
-      // from the gallop's registers, or reloaded); the others keep theirs
-      int c2 = want;
-      if (lane == ky) {
-        cur += glen;
-        head = 0;
-        c2 = gfill < 0 ? 0 : gfill;
-      }
-      obase += glen;
-      const int w2 = live ? clamp_count(endp - cur, F) : 0;
-      want = w2;
-      if (gfill < 0 || pre > 0) {
-        refill(cur, head, c2, w2);
-        wait_dma();
+        const int64_t glen = pos;
+        // list gk restarts with a window from its new position (filled
+        // above from the gallop's registers, or reloaded); the others keep
+        // theirs (list pj of a hand-over reloads the slots it gave up)
+        int c2 = want;
+        if (lane == gk) {
+          cur += glen;
+          head = 0;
+          c2 = gfill < 0 ? 0 : gfill;
+        }
+        obase += glen;
+        const int w2 = live ? clamp_count(endp - cur, F) : 0;
+        want = w2;
+        if (gfill < 0 || pre > 0) {
+          refill(cur, head, c2, w2);
+          wait_dma();
+          pre = 0;
+        }
+        // no other list holds anything (list gk ran to its end); keys
+        // without values do not chain (their equal keys need no hand-over
+        // from list to list, and their step stays as compiled without it)
+        if (!E::HAS_VAL || kz == WAVE) break;
+        gk = kz;
       }
       continue;
     }
