@@ -34,10 +34,12 @@ constexpr int WM_WAVES = 4;  // mergers per workgroup of the persistent (batched
 constexpr int WM_PLAIN_MAX = 16;
 
 // LDS words per wavefront merger with a window budget of WIN elements: the
-// windows, and the transpose of the register tile for the store (bank-padded
-// rows for multi-word elements).
+// windows, and the transpose of the register tile for the store (multi-word
+// elements: bank-padded rows; three-word pairs hold half the tile's lanes at
+// a time, so that they too fit 16 mergers in a CU's LDS).
+__host__ __device__ constexpr int wmerge_xpose_rows(int W) { return W >= 3 ? WAVE / 2 : WAVE; }
 __host__ __device__ inline size_t wmerge_lds_words_per_wave(int W, int WIN) {
-  return W == 1 ? (size_t)2 * WIN : (size_t)WIN * W + (size_t)WAVE * (WIN / WAVE + 1) * W;
+  return W == 1 ? (size_t)2 * WIN : (size_t)WIN * W + (size_t)wmerge_xpose_rows(W) * (WIN / WAVE + 1) * W;
 }
 // Workgroup LDS layout: the windows of all its mergers first (merger wv's W
 // planes of WIN words at wv * W * WIN), then their transposes. Every window
@@ -259,7 +261,8 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
   constexpr bool VEC = W == 1;
   constexpr int XC = NREG / 4;            // VEC: 16-byte chunks per lane = output rows of 256
   constexpr int XS = VEC ? NREG : NREG + 1;  // transpose row stride (padded for one-word reads)
-  constexpr int XW = WAVE * XS;           // transpose words per plane
+  constexpr int XH = VEC ? 1 : WAVE / wmerge_xpose_rows(W);  // multi-word: transposed in XH parts
+  constexpr int XW = VEC ? WAVE * XS : wmerge_xpose_rows(W) * XS;  // transpose words per plane
   static_assert(!VEC || XC == 4, "the 16-byte transpose covers 16 slots per lane");
   static_assert(!VEC || SAMP == 256, "a store row of 256 outputs holds one sample");
   // win: [list a][F] circular windows, W planes of WIN (wmerge_windows);
@@ -734,22 +737,32 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
         }
       } else {
         // transpose through LDS (rows padded to NREG + 1 words against bank
-        // conflicts) so that the one-element stores are coalesced
-#pragma unroll
-        for (int w = 0; w < W; ++w) {
-#pragma unroll
-          for (int q = 0; q < NREG; ++q) xp[w * XW + lane * XS + q] = word(v[q], w);
-        }
-        wave_lds_sync();
-        // slot of output o = u*64 + lane: lane (o / NREG), register (o % NREG)
+        // conflicts) so that the one-element stores are coalesced; pairs in
+        // two halves of the tile: output o = u*64 + lane sits in tile lane
+        // o / NREG, register o % NREG, so outputs u < NREG/2 come from tile
+        // lanes 0-31 and the others from 32-63 (row = tile lane - 32h)
+        constexpr int UH = NREG / XH;
+        constexpr int RH = WAVE / XH;  // tile lanes per part
         const int xb = (lane / NREG) * XS + (lane % NREG);
 #pragma unroll
-        for (int u = 0; u < NREG; ++u) {
-          const int slot = xb + u * (WAVE / NREG) * XS;
-          uint32_t wd[W];
+        for (int h = 0; h < XH; ++h) {
+          if (XH == 1 || lane / RH == h) {
 #pragma unroll
-          for (int w = 0; w < W; ++w) wd[w] = xp[w * XW + slot];
-          ov[u] = from_words<E>(wd);
+            for (int w = 0; w < W; ++w) {
+#pragma unroll
+              for (int q = 0; q < NREG; ++q) xp[w * XW + (lane % RH) * XS + q] = word(v[q], w);
+            }
+          }
+          wave_lds_sync();
+#pragma unroll
+          for (int u = 0; u < UH; ++u) {
+            const int slot = xb + u * (WAVE / NREG) * XS;
+            uint32_t wd[W];
+#pragma unroll
+            for (int w = 0; w < W; ++w) wd[w] = xp[w * XW + slot];
+            ov[h * UH + u] = from_words<E>(wd);
+          }
+          if (XH > 1) wave_lds_sync();  // this part's reads before the next part's writes
         }
       }
     }
