@@ -902,7 +902,9 @@ __global__ __launch_bounds__(WM_WAVES* WAVE, 4) void k_wmerge_persistent(RoundAr
   for (;;) {
     unsigned int it = 0;
     if (lane == 0) it = atomicAdd(a.queue, 1u);
-    it = (unsigned int)__shfl((int)it, 0, WAVE);
+    // wave-uniform in a scalar register, so that the work item's fields
+    // load with scalar loads and stay out of the vector registers
+    it = (unsigned int)__builtin_amdgcn_readfirstlane((int)it);
     if ((int64_t)it >= a.n_items) break;
     wmerge_one<DT, CONV_OUT, WIN, false>(a, load_merger(a.items + it), lane, win, xp, wp);
     wave_lds_sync();

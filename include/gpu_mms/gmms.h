@@ -287,9 +287,11 @@ int gmms_seg_tuning_get(int dtype, int32_t* kway, int64_t* merger_elems, int cla
 int gmms_seg_tuning_set(int dtype, const int32_t* kway, const int64_t* merger_elems, int classes);
 /* Autotuner: for each class in [min_class, max_class], sorts batches of
  * random segments of that size (about batch_elems elements per batch) with
- * every candidate K (4..64 within the LDS window) and merger size, times
- * them on `stream` with HIP events (best of `reps`) and installs the fastest
- * into the tuning table. kway_out/merger_out/ms_out (may be NULL) receive
+ * K = 16 and every candidate merger size, times them on `stream` with HIP
+ * events (best of `reps`) and installs the fastest into the tuning table;
+ * the default entry (merger size 0) stays unless a candidate is at least
+ * 3 % faster. (Another K saves a round for a class alone but costs more
+ * per element in a batch of mixed sizes; set it with gmms_seg_tuning_set.) kway_out/merger_out/ms_out (may be NULL) receive
  * the chosen entries and their time per batch, indexed by class.
  * Allocates its own buffers; synchronous. */
 int gmms_seg_autotune(int dtype, int min_class, int max_class, int64_t batch_elems, int reps, void* stream,
