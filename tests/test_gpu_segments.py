@@ -121,6 +121,23 @@ def test_pairs(gmms, oracle, dev, dist):
         assert np.array_equal(ks[a:b], wk) and np.array_equal(vs[a:b], wv), f"segment {s}"
 
 
+def test_pairs_value_is_position_handover(gmms, oracle, dev):
+    """Zipf keys over 64 values, value = input position, in segments long
+    enough for several merge rounds: the persistent merger's hand-overs and
+    chained gallops (each list's copies of a key follow the previous
+    list's)."""
+    print("segments pairs handover", flush=True)
+    offs = _offsets([1_000_003, 70_000, 2_000_000, 4097])
+    n = int(offs[-1])
+    k, v = oracle.gen_pairs(711, n, 3, 1.1, 64)
+    ks, vs = gmms.segmented_sort(_to_dev(k.view(np.int64), dev), offs, _to_dev(v.view(np.int32), dev))
+    ks, vs = ks.cpu().numpy().view(np.uint64), vs.cpu().numpy().view(np.uint32)
+    for s in range(len(offs) - 1):
+        a, b = int(offs[s]), int(offs[s + 1])
+        wk, wv = oracle.sort_pairs(k[a:b], v[a:b])
+        assert np.array_equal(ks[a:b], wk) and np.array_equal(vs[a:b], wv), f"segment {s}"
+
+
 @pytest.mark.parametrize("K", [2, 4, 64])
 def test_forced_branching_factor(gmms, oracle, dev, K):
     print(f"segments K={K}", flush=True)

@@ -147,22 +147,26 @@ def test_pairs(gmms, oracle, dev, dist):
     assert np.array_equal(_host(vs).view(np.uint32), wv)
 
 
-@pytest.mark.parametrize("distinct,n,mergers", [(64, 3_000_017, 0), (4096, 2_000_003, 101), (1 << 20, 1 << 21, 0)])
-def test_pairs_value_is_position_handover(gmms, oracle, dev, distinct, n, mergers):
+@pytest.mark.parametrize("distinct,n,mergers,desc", [(64, 3_000_017, 0, False), (4096, 2_000_003, 101, False),
+                                                     (1 << 20, 1 << 21, 0, False), (64, 3_000_017, 0, True)])
+def test_pairs_value_is_position_handover(gmms, oracle, dev, distinct, n, mergers, desc):
     """Zipf(1.1) keys over few or many distinct values, value = input
     position: equal keys are ordered by value, so each list's copies of a
     key follow the previous list's, and a merger hands over from one list
     to the next (the elements one list takes before the next list's head
-    are written ahead of that list's gallop). Split tasks (mergers = 101)
-    start inside such runs."""
+    are written ahead of that list's gallop, and gallops chain from list to
+    list). Split tasks (mergers = 101) start inside such runs; descending
+    order reverses the lists' order within each key."""
     import torch
 
-    print(f"pairs handover distinct={distinct} n={n} mergers={mergers}", flush=True)
+    print(f"pairs handover distinct={distinct} n={n} mergers={mergers} desc={desc}", flush=True)
     k, v = oracle.gen_pairs(77 + distinct, n, 3, 1.1, distinct)
-    p = gmms.make_params(K=16, mergers=mergers) if mergers else None
+    p = gmms.make_params(K=16, mergers=mergers, descending=desc) if (mergers or desc) else None
     ks, vs = gmms.sort(torch.from_numpy(k.view(np.int64)).to(dev), torch.from_numpy(v.view(np.int32)).to(dev),
                        params=p)
     wk, wv = oracle.sort_pairs(k, v)
+    if desc:
+        wk, wv = wk[::-1], wv[::-1]
     assert np.array_equal(_host(ks).view(np.uint64), wk)
     assert np.array_equal(_host(vs).view(np.uint32), wv)
 
