@@ -18,9 +18,9 @@ Reported:
   sort_roofline  the whole sort: the bytes of the passes it makes
              (2*N*4 per pass: base case + each merge round) over the step
              time, against 8 TB/s and against the best measured copy; the
-             north star's formula 2*N*4*(1+ceil(log_K(N/1024))) is reported
-             beside it (it credits a pass this build does not make when its
-             4096-element base runs save a round).
+             reference's byte formula 2*N*4*(1+ceil(log_K(N/1024))) (for its
+             1024-element base runs) is reported beside it (it credits a pass
+             that is not made when 4096-element base runs save a round).
   copy_probe every variant of the stream-copy probe (16-byte loads, 1-8 in
              flight per lane, plain or non-temporal stores, grid sizes) and
              torch's own device copy, 4 GiB each; the best is the copy peak.
@@ -524,9 +524,9 @@ def copy_probe(dev, stream, sweep=True):
 
 
 def cpu_baseline(oracle, host, threads):
-    """BASELINE's CPU plan within a bounded sample: the parallel sort of the
-    full input (median of 5 after a warm-up), one single-thread std::sort of
-    the full input, and both at N = 2^20 (median of 5 after a warm-up)."""
+    """The CPU sorts within a bounded sample: the parallel sort of the full
+    input (median of 5 after a warm-up), one single-thread std::sort of the
+    full input, and both at N = 2^20 (median of 5 after a warm-up)."""
     import platform
 
     import numpy as np
@@ -584,7 +584,7 @@ def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this build sorts on one GPU; runs on more than one GPU are not part of it "
+        print("bench.py: gpu-mms-amd sorts on one GPU; runs on more than one GPU are not part of it "
               "(--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
 
@@ -612,7 +612,7 @@ def main():
     K = resolved.kway
     kernel_name = {0: "k_wmerge (wavefront tile merge)", 1: "k_merge (pipelined minBlockHeap)",
                    2: "k_tmerge (workgroup tile merge)"}[resolved.pipeline]
-    rounds = gmms.sort_rounds(gmms.INT32, n, params)  # rounds this build runs
+    rounds = gmms.sort_rounds(gmms.INT32, n, params)  # merge rounds the sort runs
     rounds_ref = gmms.num_rounds(n, K)  # ceil(log_K(N/1024)): the reference's schedule
     # keep an untouched copy of input 0 for the bit-exact check
     check_src = base.clone()
@@ -665,8 +665,8 @@ def main():
     bit_exact = bool(np.array_equal(got, want))
     del got, want, k2, k2a
 
-    # the bytes of the passes this build makes (base case + each round), and
-    # the north star's formula 2*N*4*(1 + ceil(log_K(N/1024))) with the K that
+    # the bytes of the passes the sort makes (base case + each round), and
+    # the reference's formula 2*N*4*(1 + ceil(log_K(N/1024))) with the K that
     # ran (with 4096-element base runs it may count a pass that is not made)
     pass_bytes = 2.0 * n * 4 * (1 + rounds)
     formula_bytes = 2.0 * n * 4 * (1 + rounds_ref)
@@ -721,7 +721,7 @@ def main():
                           "formula": "2*N*4*(1+ceil(log_K(N/1024)))", "formula_bytes": formula_bytes,
                           "formula_GBps": round(formula_gbps, 1),
                           "formula_frac_of_copy_peak": round(formula_gbps / copy_peak, 4),
-                          "formula_note": ("the formula counts %d passes; this build makes %d (4096-element "
+                          "formula_note": ("the formula counts %d passes; the sort makes %d (4096-element "
                                            "base runs), so the formula figure credits %d pass(es) not made"
                                            % (1 + rounds_ref, 1 + rounds, rounds_ref - rounds))},
         "stage_ms_avg": {k: round(sum(v) / len(v), 4) for k, v in per_kind.items()},

@@ -28,7 +28,7 @@ extern "C" {
 typedef enum gmms_status {
   GMMS_SUCCESS = 0,
   GMMS_ERROR_INVALID_VALUE = 1, /* bad pointer, size, dtype or parameter */
-  GMMS_ERROR_UNSUPPORTED = 2,   /* valid request this build does not run  */
+  GMMS_ERROR_UNSUPPORTED = 2,   /* valid request the library does not run */
   GMMS_ERROR_WORKSPACE = 3,     /* workspace missing or too small         */
   GMMS_ERROR_HIP = 4,           /* a HIP runtime call or launch failed    */
   GMMS_ERROR_OUT_OF_MEMORY = 5, /* device allocation failed               */

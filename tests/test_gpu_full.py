@@ -160,7 +160,7 @@ def test_uint64_2p27(gmms, dev):
 
 
 @pytest.mark.parametrize("dist", [0, 1, 2, 3], ids=["uniform", "sorted", "reverse", "zipf1.1"])
-def test_pairs_2p27_baseline_distributions(gmms, oracle, dev, dist):
+def test_pairs_2p27_benchmark_distributions(gmms, oracle, dev, dist):
     """The benchmark's pair inputs at full size, from the oracle's own
     generators (seed 0x6D6D7310 + dist): uniform 64-bit keys, the same keys
     sorted and reverse sorted, and Zipf(1.1) over 2^20 values; value = index.
