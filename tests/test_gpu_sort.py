@@ -147,6 +147,21 @@ def test_pairs(gmms, oracle, dev, dist):
     assert np.array_equal(_host(vs).view(np.uint32), wv)
 
 
+@pytest.mark.parametrize("n", [1, 63, 64, 1023, 1025, 4095, 4097, 65537, 300_001, (1 << 20) + 3])
+def test_pairs_sizes(gmms, oracle, dev, n):
+    """Pairs across the tile, run and round boundaries (partial store rows of
+    the two-pass transpose, lists of every length), Zipf keys (ties broken by
+    value)."""
+    import torch
+
+    print(f"pairs n={n}", flush=True)
+    k, v = oracle.gen_pairs(4000 + n, n, 3, 1.1, 1 << 12)
+    ks, vs = gmms.sort(torch.from_numpy(k.view(np.int64)).to(dev), torch.from_numpy(v.view(np.int32)).to(dev))
+    wk, wv = oracle.sort_pairs(k, v)
+    assert np.array_equal(_host(ks).view(np.uint64), wk)
+    assert np.array_equal(_host(vs).view(np.uint32), wv)
+
+
 @pytest.mark.parametrize("distinct,n,mergers,desc", [(64, 3_000_017, 0, False), (4096, 2_000_003, 101, False),
                                                      (1 << 20, 1 << 21, 0, False), (64, 3_000_017, 0, True)])
 def test_pairs_value_is_position_handover(gmms, oracle, dev, distinct, n, mergers, desc):
