@@ -130,12 +130,16 @@ def test_pairs_value_is_position_handover(gmms, oracle, dev):
     offs = _offsets([1_000_003, 70_000, 2_000_000, 4097])
     n = int(offs[-1])
     k, v = oracle.gen_pairs(711, n, 3, 1.1, 64)
-    ks, vs = gmms.segmented_sort(_to_dev(k.view(np.int64), dev), offs, _to_dev(v.view(np.int32), dev))
-    ks, vs = ks.cpu().numpy().view(np.uint64), vs.cpu().numpy().view(np.uint32)
-    for s in range(len(offs) - 1):
-        a, b = int(offs[s]), int(offs[s + 1])
-        wk, wv = oracle.sort_pairs(k[a:b], v[a:b])
-        assert np.array_equal(ks[a:b], wk) and np.array_equal(vs[a:b], wv), f"segment {s}"
+    for desc in (False, True):
+        p = gmms.make_params(descending=True) if desc else None
+        ks, vs = gmms.segmented_sort(_to_dev(k.view(np.int64), dev), offs, _to_dev(v.view(np.int32), dev), params=p)
+        ks, vs = ks.cpu().numpy().view(np.uint64), vs.cpu().numpy().view(np.uint32)
+        for s in range(len(offs) - 1):
+            a, b = int(offs[s]), int(offs[s + 1])
+            wk, wv = oracle.sort_pairs(k[a:b], v[a:b])
+            if desc:
+                wk, wv = wk[::-1], wv[::-1]
+            assert np.array_equal(ks[a:b], wk) and np.array_equal(vs[a:b], wv), f"segment {s} descending={desc}"
 
 
 @pytest.mark.parametrize("K", [2, 4, 64])
