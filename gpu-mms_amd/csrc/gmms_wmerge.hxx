@@ -585,8 +585,11 @@ __device__ __forceinline__ void wmerge_one(const RoundArgs& a, const MergerInfo&
 #pragma unroll
           for (int t = 0; t < 4; ++t) {
             const int64_t idx = pos + t * WAVE + lane;
+            g[t] = t * WAVE + lane < rel ? gload<DT, false>(a.in_keys, a.in_vals, src + idx) : Limits<E>::hi();
+          }
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
             const bool valid = t * WAVE + lane < rel;
-            g[t] = valid ? gload<DT, false>(a.in_keys, a.in_vals, src + idx) : Limits<E>::hi();
             ok[t] = valid && (kz == WAVE || !lt(z, g[t]));  // g <= z
           }
           int f = 4 * WAVE;  // batch index (t*64 + lane order) of the first element not taken
